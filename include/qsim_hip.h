@@ -1,0 +1,214 @@
+/* qsim_hip.h -- C ABI of the MI355X (gfx950) state-vector backend for qsim.
+ *
+ * Plain pointers and sizes only.  State vectors are device pointers to
+ * interleaved complex numbers ("normal order": amplitude i is (s[2i], s[2i+1]))
+ * of either complex64 (QSIM_HIP_F32) or complex128 (QSIM_HIP_F64); a state of
+ * n qubits holds qsim_hip_min_size(n) real elements.  All functions return
+ * QSIM_HIP_OK (0) or a negative code; qsim_hip_last_error() describes the last
+ * failure of the calling thread.  Work is issued on the library stream
+ * (the legacy default stream unless qsim_hip_set_stream() chose another);
+ * gate applications are asynchronous, reductions and host copies synchronous.
+ *
+ * Each entry point names the reference interface it replaces (file:line in
+ * the qsim sources).  The C++ adapter in include/qsim_amd/ maps these calls
+ * onto qsim's Simulator / StateSpace / VectorSpace template API.
+ */
+#ifndef QSIM_HIP_H_
+#define QSIM_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  QSIM_HIP_OK = 0,
+  QSIM_HIP_ERR_ARG = -1,         /* invalid argument (sizes, qubits) */
+  QSIM_HIP_ERR_RUNTIME = -2,     /* HIP runtime / kernel launch failure */
+  QSIM_HIP_ERR_OOM = -3,         /* device allocation failed */
+  QSIM_HIP_ERR_UNSUPPORTED = -4  /* gate size beyond what the path supports */
+};
+
+/* Element type of a state vector. */
+enum { QSIM_HIP_F32 = 0, QSIM_HIP_F64 = 1 };
+
+/* Arithmetic used for an uncontrolled gate pass (qsim_hip_set_gate_method). */
+enum {
+  QSIM_HIP_METHOD_AUTO = 0,      /* FP32/FP64 vector kernels */
+  QSIM_HIP_METHOD_VECTOR = 1,    /* always the vector kernels */
+  QSIM_HIP_METHOD_MFMA_BF16X3 = 2,  /* k>=5: split-BF16 x3 MFMA, FP32 accum */
+  QSIM_HIP_METHOD_MFMA_FP16X2 = 3   /* k>=5: scaled two-term FP16 MFMA */
+};
+
+/* ---- context ----------------------------------------------------------- */
+
+/* Last error message of the calling thread ("" if none). */
+const char* qsim_hip_last_error(void);
+
+/* Selects the device and creates the library's per-device resources.
+ * Called implicitly by every other entry point with device 0. */
+int qsim_hip_init(int device);
+
+/* Device name, total memory (bytes) and compute-unit count. */
+int qsim_hip_device_info(char* name, size_t name_len, uint64_t* total_mem,
+                         int* num_cus);
+
+/* Issues work on `stream` (a hipStream_t; NULL = legacy default stream). */
+int qsim_hip_set_stream(void* stream);
+
+/* Waits for all work on the library stream.
+ * Replaces VectorSpaceCUDA::DeviceSync (lib/vectorspace_cuda.h:162-164). */
+int qsim_hip_synchronize(void);
+
+/* ---- VectorSpace (lib/vectorspace_cuda.h:42-167) ----------------------- */
+
+/* Real elements of an n-qubit state: 2 * 2^n, at least 64 so that tiny
+ * states keep a 256-byte footprint.  Replaces StateSpaceCUDA::MinSize
+ * (lib/statespace_cuda.h:81-83). */
+uint64_t qsim_hip_min_size(unsigned num_qubits);
+
+/* Allocates an n-qubit state (uninitialised).  On OOM returns
+ * QSIM_HIP_ERR_OOM and *out = NULL.  Replaces VectorSpaceCUDA::Create
+ * (lib/vectorspace_cuda.h:87-96). */
+int qsim_hip_alloc(int prec, unsigned num_qubits, void** out);
+
+/* Frees a pointer from qsim_hip_alloc (NULL is a no-op).  Replaces
+ * VectorSpaceCUDA::Free / detail::free (lib/vectorspace_cuda.h:31-33). */
+int qsim_hip_free(void* ptr);
+
+/* Byte copies device<-device, host<-device and device<-host.  Replace the
+ * four VectorSpaceCUDA::Copy overloads (lib/vectorspace_cuda.h:116-160). */
+int qsim_hip_copy_d2d(void* dst, const void* src, uint64_t bytes);
+int qsim_hip_copy_d2h(void* dst, const void* src, uint64_t bytes);
+int qsim_hip_copy_h2d(void* dst, const void* src, uint64_t bytes);
+
+/* ---- StateSpace (lib/statespace_cuda.h) -------------------------------- */
+
+/* All amplitudes 0 (lib/statespace_cuda.h:109-112). */
+int qsim_hip_set_all_zeros(int prec, void* state, unsigned num_qubits);
+
+/* |0...0> (lib/statespace_cuda.h:130-135). */
+int qsim_hip_set_state_zero(int prec, void* state, unsigned num_qubits);
+
+/* Uniform superposition 2^(-n/2) (lib/statespace_cuda.h:115-127). */
+int qsim_hip_set_state_uniform(int prec, void* state, unsigned num_qubits);
+
+/* Layout conversion.  The device layout is already normal order, so both
+ * are no-ops kept for API parity (lib/statespace_cuda.h:85-107). */
+int qsim_hip_internal_to_normal_order(int prec, void* state,
+                                      unsigned num_qubits);
+int qsim_hip_normal_to_internal_order(int prec, void* state,
+                                      unsigned num_qubits);
+
+/* One amplitude; thread-safe (lib/statespace_cuda.h:138-164). */
+int qsim_hip_get_ampl(int prec, const void* state, unsigned num_qubits,
+                      uint64_t i, double* re, double* im);
+int qsim_hip_set_ampl(int prec, void* state, unsigned num_qubits, uint64_t i,
+                      double re, double im);
+
+/* state[i] = re + i im where (i & mask) == bits (or != bits if exclude)
+ * (lib/statespace_cuda.h:168-187). */
+int qsim_hip_bulk_set_ampl(int prec, void* state, unsigned num_qubits,
+                           uint64_t mask, uint64_t bits, double re, double im,
+                           int exclude);
+
+/* dest += src (lib/statespace_cuda.h:190-205). */
+int qsim_hip_add(int prec, const void* src, void* dest, unsigned num_qubits);
+
+/* state *= a; `a` is rounded to the state's element type first, as the
+ * reference's fp_type argument is (lib/statespace_cuda.h:208-217). */
+int qsim_hip_multiply(int prec, void* state, unsigned num_qubits, double a);
+
+/* sum_i conj(a_i) b_i with FP64 accumulation (lib/statespace_cuda.h:219-229). */
+int qsim_hip_inner_product(int prec, const void* a, const void* b,
+                           unsigned num_qubits, double* re, double* im);
+
+/* Re sum_i conj(a_i) b_i (lib/statespace_cuda.h:231-237). */
+int qsim_hip_real_inner_product(int prec, const void* a, const void* b,
+                                unsigned num_qubits, double* out);
+
+/* sum_i |s_i|^2 in FP64 (lib/statespace_cuda.h:239-241). */
+int qsim_hip_norm(int prec, const void* state, unsigned num_qubits,
+                  double* out);
+
+/* Inverse-CDF sampling with caller-supplied draws: `draws` holds `num`
+ * ascending values in [0, norm) (lib/util.h:67-85 order); out[j] = first index
+ * k with draws[j] < sum_{i<=k} |s_i|^2 (FP64 prefix sums), or 2^n - 1 if none
+ * (lib/statespace_cuda.h:243-312, lib/statespace_basic.h:195-233). */
+int qsim_hip_sample(int prec, const void* state, unsigned num_qubits,
+                    const double* draws, uint64_t num, uint64_t* out);
+
+/* As qsim_hip_sample with the draws of GenerateRandomValues(num, seed, norm)
+ * (lib/util.h:67-85): mt19937(seed), uniform_real_distribution<double>(0,
+ * norm), sorted.  Replaces StateSpaceCUDA::Sample (lib/statespace_cuda.h:
+ * 243-312). */
+int qsim_hip_sample_seeded(int prec, const void* state, unsigned num_qubits,
+                           uint64_t num, unsigned seed, uint64_t* out);
+
+/* Number of partial norms qsim_hip_partial_norms returns for n qubits. */
+unsigned qsim_hip_num_partial_norms(unsigned num_qubits);
+
+/* Norms of qsim_hip_num_partial_norms(n) equal, contiguous index ranges
+ * (lib/statespace_cuda.h:331-355). */
+int qsim_hip_partial_norms(int prec, const void* state, unsigned num_qubits,
+                           double* out);
+
+/* First index k in partial range m with r < running sum, masked
+ * (lib/statespace_cuda.h:357-373, lib/statespace_basic.h:275-295). */
+int qsim_hip_find_measured_bits(int prec, const void* state,
+                                unsigned num_qubits, unsigned m, double r,
+                                uint64_t mask, uint64_t* out);
+
+/* Zeroes amplitudes with (i & mask) != bits and rescales the rest by
+ * 1/sqrt(their norm) (lib/statespace_cuda.h:316-329). */
+int qsim_hip_collapse(int prec, void* state, unsigned num_qubits,
+                      uint64_t mask, uint64_t bits);
+
+/* ---- Simulator (lib/simulator_cuda.h) ---------------------------------- */
+
+/* psi <- U psi.  qs: k ascending qubit indices (k <= 6); matrix: row-major
+ * interleaved complex 2^k x 2^k, row/column bit j <-> qs[j], element type
+ * matrix_prec.  k = 0 multiplies by the 1x1 matrix.  k > 6 returns
+ * QSIM_HIP_ERR_UNSUPPORTED.  Replaces SimulatorCUDA::ApplyGate
+ * (lib/simulator_cuda.h:70-125). */
+int qsim_hip_apply_gate(int prec, void* state, unsigned num_qubits,
+                        const unsigned* qs, unsigned k, const void* matrix,
+                        int matrix_prec);
+
+/* Controlled U on k <= 4 targets (any k when nc == 0), nc ascending control
+ * qubits, cvals bit i <-> cqs[i].  Replaces SimulatorCUDA::
+ * ApplyControlledGate (lib/simulator_cuda.h:135-207). */
+int qsim_hip_apply_controlled_gate(int prec, void* state, unsigned num_qubits,
+                                   const unsigned* qs, unsigned k,
+                                   const unsigned* cqs, unsigned nc,
+                                   uint64_t cvals, const void* matrix,
+                                   int matrix_prec);
+
+/* <psi|U|psi> for k <= 6 with FP64 accumulation.  Replaces SimulatorCUDA::
+ * ExpectationValue (lib/simulator_cuda.h:216-260). */
+int qsim_hip_expectation_value(int prec, const void* state,
+                               unsigned num_qubits, const unsigned* qs,
+                               unsigned k, const void* matrix,
+                               int matrix_prec, double* re, double* im);
+
+/* Arithmetic of subsequent uncontrolled k >= 5 passes on complex64 states
+ * (QSIM_HIP_METHOD_*).  No reference counterpart: qsim's GPU path has only
+ * FP32 vector arithmetic (lib/simulator_cuda_kernels.h:30-202). */
+int qsim_hip_set_gate_method(int method);
+int qsim_hip_get_gate_method(void);
+
+/* ---- timing ------------------------------------------------------------ */
+
+/* Records event `slot` (0..63) on the library stream. */
+int qsim_hip_event_record(int slot);
+
+/* Milliseconds between two recorded events (waits for the second). */
+int qsim_hip_event_elapsed_ms(int slot_start, int slot_end, float* ms);
+
+#ifdef __cplusplus
+}  /* extern "C" */
+#endif
+
+#endif  /* QSIM_HIP_H_ */

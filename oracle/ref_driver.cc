@@ -1,0 +1,216 @@
+// ref_driver.cc -- TEST INFRASTRUCTURE ONLY.
+//
+// Drives qsim's own CPU simulators, compiled from the reference headers where
+// they lie (oracle/Makefile adds the reference's lib/ to the include path), so
+// that the oracle restatement and the HIP backend can be checked against the
+// reference itself.  Output goes to oracle/_ref/ only.
+//
+//   qsim_ref -c FILE [-m avx|basic32|basic64] [-f MAX_FUSED] [-t THREADS]
+//            [-s SEED] [-o STATE.bin] [-n NUM_SAMPLES] [-x SAMPLE_SEED]
+//            [-b BITSTRINGS.bin] [-d MAXTIME] [-P PASSES]
+//   qsim_ref -R NUM -x SEED -M MAX -b DRAWS.bin     (GenerateRandomValues)
+//
+// The simulation is QSimRunner::Run with MultiQubitGateFuser
+// (lib/run_qsim.h:225-289, lib/fuser_mqubit.h:185-526) on the state of the
+// chosen backend; the final state is written in normal order
+// (InternalToNormalOrder) as interleaved complex of the backend's fp type.
+// One JSON line on stdout reports timings and sizes.
+//
+// -P PASSES runs only the first PASSES fused operations (a bounded CPU sample
+// for the benchmark's baseline leg) and reports their time.
+
+#include <unistd.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+#include "circuit_qsim_parser.h"
+#include "fuser_mqubit.h"
+#include "gate_appl.h"
+#include "io_file.h"
+#include "parfor.h"
+#include "run_qsim.h"
+#include "simmux.h"
+#include "simulator_basic.h"
+#include "util.h"
+
+namespace {
+
+struct Options {
+  std::string circuit_file;
+  std::string mode = "avx";
+  unsigned max_fused = 2;
+  unsigned threads = 1;
+  unsigned seed = 1;
+  unsigned maxtime = ~0u;
+  std::string state_out;
+  uint64_t num_samples = 0;
+  unsigned sample_seed = 0;
+  std::string bits_out;
+  uint64_t draws = 0;
+  double draws_max = 1.0;
+  long passes = -1;
+};
+
+template <typename Sim, typename FP>
+struct Factory {
+  using Simulator = Sim;
+  using StateSpace = typename Sim::StateSpace;
+  explicit Factory(unsigned nt) : nt(nt) {}
+  StateSpace CreateStateSpace() const { return StateSpace(nt); }
+  Simulator CreateSimulator() const { return Simulator(nt); }
+  unsigned nt;
+};
+
+bool WriteFile(const std::string& path, const void* p, size_t bytes) {
+  FILE* f = std::fopen(path.c_str(), "wb");
+  if (!f) return false;
+  size_t w = std::fwrite(p, 1, bytes, f);
+  std::fclose(f);
+  return w == bytes;
+}
+
+template <typename Sim, typename FP>
+int Run(const Options& opt) {
+  using namespace qsim;
+  using Fac = Factory<Sim, FP>;
+  using StateSpace = typename Fac::StateSpace;
+  using Fuser = MultiQubitGateFuser<IO>;
+  using Runner = QSimRunner<IO, Fuser, Fac>;
+
+  Circuit<Operation<FP>> circuit;
+  if (!CircuitQsimParser<IOFile>::FromFile(opt.maxtime, opt.circuit_file,
+                                           circuit)) {
+    std::fprintf(stderr, "cannot parse %s\n", opt.circuit_file.c_str());
+    return 1;
+  }
+
+  Fac factory(opt.threads);
+  StateSpace ss = factory.CreateStateSpace();
+  auto state = ss.Create(circuit.num_qubits);
+  if (ss.IsNull(state)) {
+    std::fprintf(stderr, "not enough memory\n");
+    return 1;
+  }
+  ss.SetStateZero(state);
+
+  typename Runner::Parameter param;
+  param.max_fused_size = opt.max_fused;
+  param.seed = opt.seed;
+  param.verbosity = 0;
+
+  double t0 = GetTime();
+  double t_fuse = 0, t_sim = 0;
+  size_t num_fused = 0;
+  bool ok = true;
+
+  if (opt.passes < 0) {
+    Sim sim = factory.CreateSimulator();
+    std::vector<typename StateSpace::MeasurementResult> mres;
+    t0 = GetTime();
+    ok = Runner::Run(param, circuit, ss, sim, state, mres);
+    t_sim = GetTime() - t0;
+  } else {
+    // Bounded sample: fuse the whole circuit, apply the first P fused ops.
+    Sim sim = factory.CreateSimulator();
+    double tf = GetTime();
+    auto fused = Fuser::FuseGates(param, circuit.num_qubits, circuit.ops);
+    t_fuse = GetTime() - tf;
+    num_fused = fused.size();
+    std::mt19937 rgen(opt.seed);
+    size_t p = std::min<size_t>(fused.size(), size_t(opt.passes));
+    t0 = GetTime();
+    for (size_t i = 0; i < p; ++i) {
+      ApplyGate(ss, sim, fused[i], rgen, state);
+    }
+    t_sim = GetTime() - t0;
+  }
+
+  if (!ok) {
+    std::fprintf(stderr, "simulation failed\n");
+    return 1;
+  }
+
+  ss.InternalToNormalOrder(state);
+  uint64_t amps = uint64_t{1} << circuit.num_qubits;
+  if (!opt.state_out.empty()) {
+    std::vector<FP> v(StateSpace::MinSize(circuit.num_qubits));
+    ss.Copy(state, v.data());
+    if (!WriteFile(opt.state_out, v.data(), 2 * amps * sizeof(FP))) return 1;
+  }
+
+  double t_samp = 0;
+  if (opt.num_samples > 0) {
+    // Sample on the normal-order copy so the walk follows amplitude order.
+    using Basic = SimulatorBasic<ParallelFor, FP>;
+    using BSS = typename Basic::StateSpace;
+    BSS bss(1);
+    auto bstate = bss.Create(circuit.num_qubits);
+    ss.Copy(state, bstate.get());
+    double ts = GetTime();
+    auto bits = bss.Sample(bstate, opt.num_samples, opt.sample_seed);
+    t_samp = GetTime() - ts;
+    if (!opt.bits_out.empty() &&
+        !WriteFile(opt.bits_out, bits.data(), bits.size() * sizeof(uint64_t)))
+      return 1;
+  }
+
+  double norm = ss.Norm(state);
+  std::printf(
+      "{\"mode\": \"%s\", \"num_qubits\": %u, \"num_gates\": %zu, "
+      "\"max_fused\": %u, \"threads\": %u, \"fused_ops\": %zu, "
+      "\"passes\": %ld, \"fuse_s\": %.9g, \"sim_s\": %.9g, "
+      "\"sample_s\": %.9g, \"norm\": %.17g}\n",
+      opt.mode.c_str(), circuit.num_qubits, circuit.ops.size(),
+      opt.max_fused, opt.threads, num_fused, opt.passes, t_fuse, t_sim, t_samp,
+      norm);
+  return 0;
+}
+
+}  // namespace
+
+int main(int argc, char* argv[]) {
+  Options opt;
+  int k;
+  while ((k = getopt(argc, argv, "c:m:f:t:s:o:n:x:b:d:R:M:P:")) != -1) {
+    switch (k) {
+      case 'c': opt.circuit_file = optarg; break;
+      case 'm': opt.mode = optarg; break;
+      case 'f': opt.max_fused = std::atoi(optarg); break;
+      case 't': opt.threads = std::atoi(optarg); break;
+      case 's': opt.seed = std::atoi(optarg); break;
+      case 'o': opt.state_out = optarg; break;
+      case 'n': opt.num_samples = std::strtoull(optarg, nullptr, 10); break;
+      case 'x': opt.sample_seed = std::atoi(optarg); break;
+      case 'b': opt.bits_out = optarg; break;
+      case 'd': opt.maxtime = std::atoi(optarg); break;
+      case 'R': opt.draws = std::strtoull(optarg, nullptr, 10); break;
+      case 'M': opt.draws_max = std::strtod(optarg, nullptr); break;
+      case 'P': opt.passes = std::atol(optarg); break;
+      default:
+        std::fprintf(stderr, "bad option\n");
+        return 1;
+    }
+  }
+
+  if (opt.draws > 0) {
+    auto rs = qsim::GenerateRandomValues<double>(opt.draws, opt.sample_seed,
+                                                 opt.draws_max);
+    return WriteFile(opt.bits_out, rs.data(), rs.size() * sizeof(double)) ? 0
+                                                                          : 1;
+  }
+
+  if (opt.circuit_file.empty()) {
+    std::fprintf(stderr, "no circuit file\n");
+    return 1;
+  }
+
+  using For = qsim::ParallelFor;
+  if (opt.mode == "avx") return Run<qsim::Simulator<For>, float>(opt);
+  if (opt.mode == "basic32") return Run<qsim::SimulatorBasic<For, float>, float>(opt);
+  if (opt.mode == "basic64") return Run<qsim::SimulatorBasic<For, double>, double>(opt);
+  std::fprintf(stderr, "unknown mode %s\n", opt.mode.c_str());
+  return 1;
+}

@@ -1,0 +1,563 @@
+// api.hip -- the C ABI of include/qsim_hip.h: context, memory, dispatch.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "internal.h"
+
+namespace qhip {
+
+namespace {
+
+thread_local char g_error[512] = "";
+
+struct Device {
+  bool ready = false;
+  int id = 0;
+  int cus = 0;
+  void* scratch = nullptr;
+  size_t scratch_size = 0;
+  // Matrix ring: device slots and pinned host staging, one event per slot.
+  static constexpr int kSlots = 64;
+  static constexpr size_t kSlotBytes = 64 * 1024;  // k = 6 FP64 matrix
+  char* mat_dev = nullptr;
+  char* mat_host = nullptr;
+  hipEvent_t mat_ev[kSlots] = {};
+  bool mat_ev_used[kSlots] = {};
+  int mat_next = 0;
+  hipEvent_t ev[64] = {};
+};
+
+std::mutex g_init_mu;
+std::mutex g_scratch_mu;
+std::mutex g_mat_mu;
+Device g_dev;
+hipStream_t g_stream = nullptr;
+std::atomic<int> g_method{QSIM_HIP_METHOD_AUTO};
+
+}  // namespace
+
+void SetError(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(g_error, sizeof(g_error), fmt, ap);
+  va_end(ap);
+}
+
+int HipFail(hipError_t e, const char* what) {
+  SetError("%s: %s", what, hipGetErrorString(e));
+  return e == hipErrorOutOfMemory ? QSIM_HIP_ERR_OOM : QSIM_HIP_ERR_RUNTIME;
+}
+
+static int InitDevice(int device) {
+  std::lock_guard<std::mutex> lock(g_init_mu);
+  if (g_dev.ready) {
+    if (device != g_dev.id) {
+      SetError("library already initialised on device %d", g_dev.id);
+      return QSIM_HIP_ERR_ARG;
+    }
+    return QSIM_HIP_OK;
+  }
+  int count = 0;
+  QHIP_CHECK(hipGetDeviceCount(&count));
+  if (device < 0 || device >= count) {
+    SetError("device %d not present (%d devices)", device, count);
+    return QSIM_HIP_ERR_ARG;
+  }
+  QHIP_CHECK(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  QHIP_CHECK(hipGetDeviceProperties(&prop, device));
+  g_dev.id = device;
+  g_dev.cus = prop.multiProcessorCount;
+  QHIP_CHECK(hipMalloc(&g_dev.mat_dev, Device::kSlots * Device::kSlotBytes));
+  QHIP_CHECK(hipHostMalloc(&g_dev.mat_host, Device::kSlots * Device::kSlotBytes,
+                           hipHostMallocDefault));
+  for (int i = 0; i < Device::kSlots; ++i) {
+    QHIP_CHECK(hipEventCreateWithFlags(&g_dev.mat_ev[i],
+                                       hipEventDisableTiming));
+  }
+  for (int i = 0; i < 64; ++i) QHIP_CHECK(hipEventCreate(&g_dev.ev[i]));
+  g_dev.ready = true;
+  return QSIM_HIP_OK;
+}
+
+int EnsureInit() {
+  if (g_dev.ready) return QSIM_HIP_OK;
+  return InitDevice(0);
+}
+
+hipStream_t Stream() { return g_stream; }
+
+int NumCUs() { return g_dev.cus; }
+
+std::mutex& ScratchLock() { return g_scratch_mu; }
+
+void* DeviceScratch(size_t bytes) {
+  if (bytes <= g_dev.scratch_size) return g_dev.scratch;
+  if (g_dev.scratch != nullptr) {
+    // Earlier work may still read the old buffer.
+    if (hipStreamSynchronize(g_stream) != hipSuccess ||
+        hipFree(g_dev.scratch) != hipSuccess) {
+      SetError("cannot release scratch buffer");
+      return nullptr;
+    }
+    g_dev.scratch = nullptr;
+    g_dev.scratch_size = 0;
+  }
+  size_t sz = std::max<size_t>(bytes, 1 << 20);
+  hipError_t e = hipMalloc(&g_dev.scratch, sz);
+  if (e != hipSuccess) {
+    HipFail(e, "hipMalloc(scratch)");
+    g_dev.scratch = nullptr;
+    return nullptr;
+  }
+  g_dev.scratch_size = sz;
+  return g_dev.scratch;
+}
+
+void* MatrixBuffer(size_t bytes) {
+  if (bytes > Device::kSlotBytes) {
+    SetError("matrix of %zu bytes exceeds the staging slot", bytes);
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_mat_mu);
+  int slot = g_dev.mat_next;
+  g_dev.mat_next = (slot + 1) % Device::kSlots;
+  return g_dev.mat_dev + size_t(slot) * Device::kSlotBytes;
+}
+
+int StageMatrix(void* dev, const void* host, size_t bytes) {
+  std::lock_guard<std::mutex> lock(g_mat_mu);
+  size_t slot = (static_cast<char*>(dev) - g_dev.mat_dev) / Device::kSlotBytes;
+  char* h = g_dev.mat_host + slot * Device::kSlotBytes;
+  // The pinned slot may still be the source of an earlier, unfinished copy.
+  if (g_dev.mat_ev_used[slot])
+    QHIP_CHECK(hipEventSynchronize(g_dev.mat_ev[slot]));
+  std::memcpy(h, host, bytes);
+  QHIP_CHECK(hipMemcpyAsync(dev, h, bytes, hipMemcpyHostToDevice, g_stream));
+  QHIP_CHECK(hipEventRecord(g_dev.mat_ev[slot], g_stream));
+  g_dev.mat_ev_used[slot] = true;
+  return QSIM_HIP_OK;
+}
+
+}  // namespace qhip
+
+using namespace qhip;
+
+namespace {
+
+int CheckState(int prec, const void* s, unsigned n) {
+  if (prec != QSIM_HIP_F32 && prec != QSIM_HIP_F64) {
+    SetError("unknown precision %d", prec);
+    return QSIM_HIP_ERR_ARG;
+  }
+  if (s == nullptr) {
+    SetError("null state");
+    return QSIM_HIP_ERR_ARG;
+  }
+  if (n > 40) {
+    SetError("%u qubits exceed the supported 40", n);
+    return QSIM_HIP_ERR_ARG;
+  }
+  return EnsureInit();
+}
+
+uint64_t ElemBytes(int prec) { return prec == QSIM_HIP_F32 ? 4 : 8; }
+
+}  // namespace
+
+extern "C" {
+
+const char* qsim_hip_last_error(void) { return g_error; }
+
+int qsim_hip_init(int device) { return InitDevice(device); }
+
+int qsim_hip_device_info(char* name, size_t name_len, uint64_t* total_mem,
+                         int* num_cus) {
+  QHIP_TRY(EnsureInit());
+  hipDeviceProp_t prop;
+  QHIP_CHECK(hipGetDeviceProperties(&prop, g_dev.id));
+  if (name && name_len) {
+    std::snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+  }
+  if (total_mem) *total_mem = prop.totalGlobalMem;
+  if (num_cus) *num_cus = prop.multiProcessorCount;
+  return QSIM_HIP_OK;
+}
+
+int qsim_hip_set_stream(void* stream) {
+  QHIP_TRY(EnsureInit());
+  g_stream = static_cast<hipStream_t>(stream);
+  return QSIM_HIP_OK;
+}
+
+int qsim_hip_synchronize(void) {
+  QHIP_TRY(EnsureInit());
+  QHIP_CHECK(hipStreamSynchronize(g_stream));
+  return QSIM_HIP_OK;
+}
+
+uint64_t qsim_hip_min_size(unsigned num_qubits) {
+  return std::max<uint64_t>(64, uint64_t{2} << num_qubits);
+}
+
+int qsim_hip_alloc(int prec, unsigned num_qubits, void** out) {
+  if (out == nullptr) return QSIM_HIP_ERR_ARG;
+  *out = nullptr;
+  if (num_qubits > 40) {
+    SetError("%u qubits exceed the supported 40", num_qubits);
+    return QSIM_HIP_ERR_ARG;
+  }
+  QHIP_TRY(EnsureInit());
+  uint64_t bytes = ElemBytes(prec) * qsim_hip_min_size(num_qubits);
+  hipError_t e = hipMalloc(out, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *out = nullptr;
+    SetError("hipMalloc(%llu bytes): %s", (unsigned long long)bytes,
+             hipGetErrorString(e));
+    return QSIM_HIP_ERR_OOM;
+  }
+  return QSIM_HIP_OK;
+}
+
+int qsim_hip_free(void* ptr) {
+  if (ptr == nullptr) return QSIM_HIP_OK;
+  QHIP_TRY(EnsureInit());
+  QHIP_CHECK(hipStreamSynchronize(g_stream));
+  QHIP_CHECK(hipFree(ptr));
+  return QSIM_HIP_OK;
+}
+
+static int Copy(void* dst, const void* src, uint64_t bytes, hipMemcpyKind k) {
+  QHIP_TRY(EnsureInit());
+  if (bytes == 0) return QSIM_HIP_OK;
+  if (dst == nullptr || src == nullptr) {
+    SetError("null copy operand");
+    return QSIM_HIP_ERR_ARG;
+  }
+  QHIP_CHECK(hipMemcpyAsync(dst, src, bytes, k, g_stream));
+  QHIP_CHECK(hipStreamSynchronize(g_stream));
+  return QSIM_HIP_OK;
+}
+
+int qsim_hip_copy_d2d(void* dst, const void* src, uint64_t bytes) {
+  return Copy(dst, src, bytes, hipMemcpyDeviceToDevice);
+}
+int qsim_hip_copy_d2h(void* dst, const void* src, uint64_t bytes) {
+  return Copy(dst, src, bytes, hipMemcpyDeviceToHost);
+}
+int qsim_hip_copy_h2d(void* dst, const void* src, uint64_t bytes) {
+  return Copy(dst, src, bytes, hipMemcpyHostToDevice);
+}
+
+#define QHIP_DISPATCH(prec, F32CALL, F64CALL) \
+  ((prec) == QSIM_HIP_F32 ? (F32CALL) : (F64CALL))
+
+int qsim_hip_set_all_zeros(int prec, void* s, unsigned n) {
+  QHIP_TRY(CheckState(prec, s, n));
+  return QHIP_DISPATCH(prec, Fill<float>((float2*)s, n, 0),
+                       Fill<double>((double2*)s, n, 0));
+}
+
+int qsim_hip_set_state_zero(int prec, void* s, unsigned n) {
+  QHIP_TRY(CheckState(prec, s, n));
+  return QHIP_DISPATCH(prec, Fill<float>((float2*)s, n, 1),
+                       Fill<double>((double2*)s, n, 1));
+}
+
+int qsim_hip_set_state_uniform(int prec, void* s, unsigned n) {
+  QHIP_TRY(CheckState(prec, s, n));
+  return QHIP_DISPATCH(prec, Fill<float>((float2*)s, n, 2),
+                       Fill<double>((double2*)s, n, 2));
+}
+
+int qsim_hip_internal_to_normal_order(int prec, void* s, unsigned n) {
+  return CheckState(prec, s, n);
+}
+
+int qsim_hip_normal_to_internal_order(int prec, void* s, unsigned n) {
+  return CheckState(prec, s, n);
+}
+
+int qsim_hip_get_ampl(int prec, const void* s, unsigned n, uint64_t i,
+                      double* re, double* im) {
+  QHIP_TRY(CheckState(prec, s, n));
+  if (i >> n) {
+    SetError("amplitude index out of range");
+    return QSIM_HIP_ERR_ARG;
+  }
+  // A blocking copy on the legacy stream: safe from concurrent host threads.
+  if (prec == QSIM_HIP_F32) {
+    float2 v;
+    QHIP_CHECK(hipStreamSynchronize(g_stream));
+    QHIP_CHECK(hipMemcpy(&v, (const float2*)s + i, sizeof(v),
+                         hipMemcpyDeviceToHost));
+    *re = v.x;
+    *im = v.y;
+  } else {
+    double2 v;
+    QHIP_CHECK(hipStreamSynchronize(g_stream));
+    QHIP_CHECK(hipMemcpy(&v, (const double2*)s + i, sizeof(v),
+                         hipMemcpyDeviceToHost));
+    *re = v.x;
+    *im = v.y;
+  }
+  return QSIM_HIP_OK;
+}
+
+int qsim_hip_set_ampl(int prec, void* s, unsigned n, uint64_t i, double re,
+                      double im) {
+  QHIP_TRY(CheckState(prec, s, n));
+  if (i >> n) {
+    SetError("amplitude index out of range");
+    return QSIM_HIP_ERR_ARG;
+  }
+  QHIP_CHECK(hipStreamSynchronize(g_stream));
+  if (prec == QSIM_HIP_F32) {
+    float2 v{float(re), float(im)};
+    QHIP_CHECK(hipMemcpy((float2*)s + i, &v, sizeof(v), hipMemcpyHostToDevice));
+  } else {
+    double2 v{re, im};
+    QHIP_CHECK(
+        hipMemcpy((double2*)s + i, &v, sizeof(v), hipMemcpyHostToDevice));
+  }
+  return QSIM_HIP_OK;
+}
+
+int qsim_hip_bulk_set_ampl(int prec, void* s, unsigned n, uint64_t mask,
+                           uint64_t bits, double re, double im, int exclude) {
+  QHIP_TRY(CheckState(prec, s, n));
+  return QHIP_DISPATCH(
+      prec, BulkSet<float>((float2*)s, n, mask, bits, re, im, exclude != 0),
+      BulkSet<double>((double2*)s, n, mask, bits, re, im, exclude != 0));
+}
+
+int qsim_hip_add(int prec, const void* src, void* dst, unsigned n) {
+  QHIP_TRY(CheckState(prec, src, n));
+  QHIP_TRY(CheckState(prec, dst, n));
+  return QHIP_DISPATCH(prec, Add<float>((const float2*)src, (float2*)dst, n),
+                       Add<double>((const double2*)src, (double2*)dst, n));
+}
+
+int qsim_hip_multiply(int prec, void* s, unsigned n, double a) {
+  QHIP_TRY(CheckState(prec, s, n));
+  return QHIP_DISPATCH(prec, Scale<float>((float2*)s, n, a),
+                       Scale<double>((double2*)s, n, a));
+}
+
+int qsim_hip_inner_product(int prec, const void* a, const void* b, unsigned n,
+                           double* re, double* im) {
+  QHIP_TRY(CheckState(prec, a, n));
+  QHIP_TRY(CheckState(prec, b, n));
+  return QHIP_DISPATCH(
+      prec, Dot<float>((const float2*)a, (const float2*)b, n, false, re, im),
+      Dot<double>((const double2*)a, (const double2*)b, n, false, re, im));
+}
+
+int qsim_hip_real_inner_product(int prec, const void* a, const void* b,
+                                unsigned n, double* out) {
+  QHIP_TRY(CheckState(prec, a, n));
+  QHIP_TRY(CheckState(prec, b, n));
+  return QHIP_DISPATCH(
+      prec, Dot<float>((const float2*)a, (const float2*)b, n, true, out, nullptr),
+      Dot<double>((const double2*)a, (const double2*)b, n, true, out, nullptr));
+}
+
+int qsim_hip_norm(int prec, const void* s, unsigned n, double* out) {
+  return qsim_hip_real_inner_product(prec, s, s, n, out);
+}
+
+int qsim_hip_sample(int prec, const void* s, unsigned n, const double* draws,
+                    uint64_t num, uint64_t* out) {
+  QHIP_TRY(CheckState(prec, s, n));
+  if (num > 0 && (draws == nullptr || out == nullptr)) {
+    SetError("null sample buffers");
+    return QSIM_HIP_ERR_ARG;
+  }
+  if (num > (uint64_t{1} << 31)) {
+    SetError("too many samples");
+    return QSIM_HIP_ERR_ARG;
+  }
+  return QHIP_DISPATCH(
+      prec, Sample<float>((const float2*)s, n, draws, num, out),
+      Sample<double>((const double2*)s, n, draws, num, out));
+}
+
+int qsim_hip_sample_seeded(int prec, const void* s, unsigned n, uint64_t num,
+                           unsigned seed, uint64_t* out) {
+  if (num == 0) return CheckState(prec, s, n);
+  double norm = 0;
+  QHIP_TRY(qsim_hip_norm(prec, s, n, &norm));
+  // The draws of lib/util.h:67-85 (mt19937, uniform_real_distribution, sort).
+  std::vector<double> rs;
+  rs.reserve(num + 1);
+  std::mt19937 rgen(seed);
+  std::uniform_real_distribution<double> distr(0.0, norm);
+  for (uint64_t i = 0; i < num; ++i) rs.push_back(distr(rgen));
+  std::sort(rs.begin(), rs.end());
+  rs.push_back(norm);
+  return qsim_hip_sample(prec, s, n, rs.data(), num, out);
+}
+
+unsigned qsim_hip_num_partial_norms(unsigned n) { return PartialNormCount(n); }
+
+int qsim_hip_partial_norms(int prec, const void* s, unsigned n, double* out) {
+  QHIP_TRY(CheckState(prec, s, n));
+  return QHIP_DISPATCH(prec, PartialNorms<float>((const float2*)s, n, out),
+                       PartialNorms<double>((const double2*)s, n, out));
+}
+
+int qsim_hip_find_measured_bits(int prec, const void* s, unsigned n,
+                                unsigned m, double r, uint64_t mask,
+                                uint64_t* out) {
+  QHIP_TRY(CheckState(prec, s, n));
+  return QHIP_DISPATCH(
+      prec, FindMeasuredBits<float>((const float2*)s, n, m, r, mask, out),
+      FindMeasuredBits<double>((const double2*)s, n, m, r, mask, out));
+}
+
+int qsim_hip_collapse(int prec, void* s, unsigned n, uint64_t mask,
+                      uint64_t bits) {
+  QHIP_TRY(CheckState(prec, s, n));
+  return QHIP_DISPATCH(prec, Collapse<float>((float2*)s, n, mask, bits),
+                       Collapse<double>((double2*)s, n, mask, bits));
+}
+
+int qsim_hip_apply_controlled_gate(int prec, void* s, unsigned n,
+                                   const unsigned* qs, unsigned k,
+                                   const unsigned* cqs, unsigned nc,
+                                   uint64_t cvals, const void* matrix,
+                                   int matrix_prec) {
+  QHIP_TRY(CheckState(prec, s, n));
+  if ((k > 0 && qs == nullptr) || (nc > 0 && cqs == nullptr) ||
+      matrix == nullptr) {
+    SetError("null gate operand");
+    return QSIM_HIP_ERR_ARG;
+  }
+  if (k > 6) {
+    SetError("gates on %u > 6 qubits are not supported", k);
+    return QSIM_HIP_ERR_UNSUPPORTED;
+  }
+  const size_t melems = size_t{2} << (2 * k);
+  int method = g_method.load();
+  if (prec == QSIM_HIP_F32 && nc == 0 && k >= 5 &&
+      (method == QSIM_HIP_METHOD_MFMA_BF16X3 ||
+       method == QSIM_HIP_METHOD_MFMA_FP16X2)) {
+    std::vector<double> m64(melems);
+    if (matrix_prec == QSIM_HIP_F64) {
+      std::memcpy(m64.data(), matrix, melems * sizeof(double));
+    } else {
+      const float* mf = static_cast<const float*>(matrix);
+      for (size_t i = 0; i < melems; ++i) m64[i] = mf[i];
+    }
+    int rc = LaunchGateMfma((float2*)s, n, qs, k, m64.data(), method);
+    if (rc != QSIM_HIP_ERR_UNSUPPORTED) return rc;
+  }
+  if (prec == QSIM_HIP_F32) {
+    std::vector<float> mf;
+    const float* m = static_cast<const float*>(matrix);
+    if (matrix_prec == QSIM_HIP_F64) {
+      mf.resize(melems);
+      const double* md = static_cast<const double*>(matrix);
+      for (size_t i = 0; i < melems; ++i) mf[i] = float(md[i]);
+      m = mf.data();
+    }
+    return LaunchGate<float>((float2*)s, n, qs, k, cqs, nc, cvals, m);
+  } else {
+    std::vector<double> md;
+    const double* m = static_cast<const double*>(matrix);
+    if (matrix_prec == QSIM_HIP_F32) {
+      md.resize(melems);
+      const float* mf = static_cast<const float*>(matrix);
+      for (size_t i = 0; i < melems; ++i) md[i] = mf[i];
+      m = md.data();
+    }
+    return LaunchGate<double>((double2*)s, n, qs, k, cqs, nc, cvals, m);
+  }
+}
+
+int qsim_hip_apply_gate(int prec, void* s, unsigned n, const unsigned* qs,
+                        unsigned k, const void* matrix, int matrix_prec) {
+  return qsim_hip_apply_controlled_gate(prec, s, n, qs, k, nullptr, 0, 0,
+                                        matrix, matrix_prec);
+}
+
+int qsim_hip_expectation_value(int prec, const void* s, unsigned n,
+                               const unsigned* qs, unsigned k,
+                               const void* matrix, int matrix_prec, double* re,
+                               double* im) {
+  QHIP_TRY(CheckState(prec, s, n));
+  if (qs == nullptr || matrix == nullptr || re == nullptr || im == nullptr) {
+    SetError("null operand");
+    return QSIM_HIP_ERR_ARG;
+  }
+  if (k > 6) {
+    SetError("operators on %u > 6 qubits are not supported", k);
+    return QSIM_HIP_ERR_UNSUPPORTED;
+  }
+  const size_t melems = size_t{2} << (2 * k);
+  if (prec == QSIM_HIP_F32) {
+    std::vector<float> mf;
+    const float* m = static_cast<const float*>(matrix);
+    if (matrix_prec == QSIM_HIP_F64) {
+      mf.resize(melems);
+      for (size_t i = 0; i < melems; ++i)
+        mf[i] = float(static_cast<const double*>(matrix)[i]);
+      m = mf.data();
+    }
+    return LaunchExpectation<float>((const float2*)s, n, qs, k, m, re, im);
+  } else {
+    std::vector<double> md;
+    const double* m = static_cast<const double*>(matrix);
+    if (matrix_prec == QSIM_HIP_F32) {
+      md.resize(melems);
+      for (size_t i = 0; i < melems; ++i)
+        md[i] = static_cast<const float*>(matrix)[i];
+      m = md.data();
+    }
+    return LaunchExpectation<double>((const double2*)s, n, qs, k, m, re, im);
+  }
+}
+
+int qsim_hip_set_gate_method(int method) {
+  if (method < QSIM_HIP_METHOD_AUTO || method > QSIM_HIP_METHOD_MFMA_FP16X2) {
+    SetError("unknown gate method %d", method);
+    return QSIM_HIP_ERR_ARG;
+  }
+  g_method.store(method);
+  return QSIM_HIP_OK;
+}
+
+int qsim_hip_get_gate_method(void) { return g_method.load(); }
+
+int qsim_hip_event_record(int slot) {
+  QHIP_TRY(EnsureInit());
+  if (slot < 0 || slot >= 64) {
+    SetError("event slot %d out of range", slot);
+    return QSIM_HIP_ERR_ARG;
+  }
+  QHIP_CHECK(hipEventRecord(g_dev.ev[slot], g_stream));
+  return QSIM_HIP_OK;
+}
+
+int qsim_hip_event_elapsed_ms(int a, int b, float* ms) {
+  QHIP_TRY(EnsureInit());
+  if (a < 0 || a >= 64 || b < 0 || b >= 64 || ms == nullptr) {
+    SetError("bad event slots");
+    return QSIM_HIP_ERR_ARG;
+  }
+  QHIP_CHECK(hipEventSynchronize(g_dev.ev[b]));
+  QHIP_CHECK(hipEventElapsedTime(ms, g_dev.ev[a], g_dev.ev[b]));
+  return QSIM_HIP_OK;
+}
+
+}  // extern "C"

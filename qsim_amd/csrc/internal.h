@@ -1,0 +1,152 @@
+// internal.h -- shared definitions of the gfx950 state-vector backend.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+
+#include "qsim_hip.h"
+
+namespace qhip {
+
+// ---- error reporting -------------------------------------------------------
+
+void SetError(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
+int HipFail(hipError_t e, const char* what);
+
+#define QHIP_CHECK(expr)                                   \
+  do {                                                     \
+    hipError_t e_ = (expr);                                \
+    if (e_ != hipSuccess) return ::qhip::HipFail(e_, #expr); \
+  } while (0)
+
+#define QHIP_LAUNCHED(name)                                   \
+  do {                                                        \
+    hipError_t e_ = hipGetLastError();                        \
+    if (e_ != hipSuccess) return ::qhip::HipFail(e_, name);   \
+  } while (0)
+
+#define QHIP_TRY(expr)            \
+  do {                            \
+    int rc_ = (expr);             \
+    if (rc_ != QSIM_HIP_OK) return rc_; \
+  } while (0)
+
+// ---- context ---------------------------------------------------------------
+
+int EnsureInit();
+hipStream_t Stream();
+
+// Grow-only device scratch of the current device (not re-entrant: callers
+// hold ScratchLock()).  Returns nullptr and sets the error on failure.
+void* DeviceScratch(size_t bytes);
+std::mutex& ScratchLock();
+
+// Device buffer for gate matrices too large to travel as kernel arguments.
+void* MatrixBuffer(size_t bytes);
+
+// Copies `bytes` of a host matrix into the device slot `dev` (from
+// MatrixBuffer) through pinned staging, ordered on the library stream.
+int StageMatrix(void* dev, const void* host, size_t bytes);
+
+int NumCUs();
+
+// ---- complex element types -------------------------------------------------
+
+template <typename FP> struct Cplx;
+template <> struct Cplx<float> { using T = float2; };
+template <> struct Cplx<double> { using T = double2; };
+
+struct Dbl2 {
+  double x, y;
+  __host__ __device__ Dbl2() : x(0), y(0) {}
+  __host__ __device__ Dbl2(double a, double b) : x(a), y(b) {}
+  __host__ __device__ Dbl2& operator+=(const Dbl2& o) {
+    x += o.x;
+    y += o.y;
+    return *this;
+  }
+};
+
+constexpr unsigned kMaxInserted = 40;  // targets + controls per gate
+
+// Geometry of a gate pass in the register ("high target") kernels: thread
+// item t -> group base = t with zero bits inserted at ins[0..nins) (ascending)
+// OR cbits; group member j at base + sum_b bit(j,b) * tbit[b].
+struct GateGeom {
+  uint64_t ngroups;
+  uint64_t cbits;
+  uint64_t tbit[6];
+  unsigned nins;
+  unsigned ins[kMaxInserted];
+};
+
+// Geometry of the LDS tile kernels.  A tile holds 2^tb amplitudes: the Lc
+// lowest qubits (contiguous in memory) and the h targets at or above Lc.
+// Tile index -> base = (tile << Lc) with zero bits inserted at the outer
+// positions (high targets and high controls) OR cbits_high.
+struct TileGeom {
+  uint64_t ntiles;
+  uint64_t cbits_high;
+  uint64_t hbit[6];
+  unsigned nouter;
+  unsigned outer[kMaxInserted];
+  unsigned Lc;
+  unsigned h;
+  unsigned tb;
+  unsigned tpos[6];     // tile-local target positions, ascending
+  unsigned low_cmask;   // controls below Lc (tile-local == global bits)
+  unsigned low_cvals;
+};
+
+__host__ __device__ inline uint64_t InsertZeros(uint64_t i, const unsigned* pos,
+                                                unsigned k) {
+  for (unsigned j = 0; j < k; ++j) {
+    uint64_t low = i & ((uint64_t{1} << pos[j]) - 1);
+    i = ((i >> pos[j]) << (pos[j] + 1)) | low;
+  }
+  return i;
+}
+
+// Host-side launch helpers implemented in gate_kernels.hip / state_kernels.hip.
+template <typename FP>
+int LaunchGate(typename Cplx<FP>::T* s, unsigned n, const unsigned* qs,
+               unsigned k, const unsigned* cqs, unsigned nc, uint64_t cvals,
+               const FP* m);
+
+template <typename FP>
+int LaunchExpectation(const typename Cplx<FP>::T* s, unsigned n,
+                      const unsigned* qs, unsigned k, const FP* m, double* re,
+                      double* im);
+
+template <typename FP> using CT = typename Cplx<FP>::T;
+
+// State-space passes (state_kernels.hip).  Fill: 0 zeros, 1 |0>, 2 uniform.
+template <typename FP> int Fill(CT<FP>* s, unsigned n, int what);
+template <typename FP>
+int BulkSet(CT<FP>* s, unsigned n, uint64_t mask, uint64_t bits, double re,
+            double im, bool exclude);
+template <typename FP> int Add(const CT<FP>* src, CT<FP>* dst, unsigned n);
+template <typename FP> int Scale(CT<FP>* s, unsigned n, double a);
+template <typename FP>
+int Dot(const CT<FP>* a, const CT<FP>* b, unsigned n, bool real, double* re,
+        double* im);
+unsigned PartialNormCount(unsigned n);
+template <typename FP> int PartialNorms(const CT<FP>* s, unsigned n, double* out);
+template <typename FP>
+int FindMeasuredBits(const CT<FP>* s, unsigned n, unsigned m, double r,
+                     uint64_t mask, uint64_t* out);
+template <typename FP>
+int Collapse(CT<FP>* s, unsigned n, uint64_t mask, uint64_t bits);
+template <typename FP>
+int Sample(const CT<FP>* s, unsigned n, const double* draws, uint64_t num,
+           uint64_t* out);
+
+// MFMA path for uncontrolled k>=5 gates on complex64 states.  `m64` is the
+// matrix in FP64 (rounded products are formed from it); returns
+// QSIM_HIP_ERR_UNSUPPORTED when the method cannot handle the gate.
+int LaunchGateMfma(float2* s, unsigned n, const unsigned* qs, unsigned k,
+                   const double* m64, int method);
+
+}  // namespace qhip
