@@ -207,6 +207,15 @@ int qsim_hip_event_record(int slot);
 /* Milliseconds between two recorded events (waits for the second). */
 int qsim_hip_event_elapsed_ms(int slot_start, int slot_end, float* ms);
 
+/* Per-pass timer: while started, every gate / controlled-gate pass records a
+ * HIP event pair around its kernel on the library stream (up to `capacity`
+ * passes).  qsim_hip_pass_timer_stop waits for the work, writes the kernel
+ * durations in ms (in launch order) and stops recording.  No reference
+ * counterpart: qsim times passes on the host with a device sync per gate
+ * (lib/run_qsim.h:129-142). */
+int qsim_hip_pass_timer_start(unsigned capacity);
+int qsim_hip_pass_timer_stop(float* ms, unsigned max_count, unsigned* count);
+
 #ifdef __cplusplus
 }  /* extern "C" */
 #endif

@@ -1,0 +1,57 @@
+/* qsim_hip_driver.h -- C ABI of the circuit driver built on qsim's own host
+ * code (circuit parser, MultiQubitGateFuser, gate dispatch) and the gfx950
+ * backend.  It is what bench.py and the tests call to run whole circuits.
+ *
+ * A circuit handle owns the parsed circuit (in FP32 and FP64 gate matrices)
+ * and the current fused partition.  States are device pointers from
+ * qsim_hip_alloc.
+ */
+#ifndef QSIM_HIP_DRIVER_H_
+#define QSIM_HIP_DRIVER_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Fuser selection for qsim_driver_fuse. */
+enum {
+  QSIM_DRIVER_FUSER_MQUBIT = 0,  /* qsim's MultiQubitGateFuser */
+  QSIM_DRIVER_FUSER_LEARNED = 1  /* cost-model fuser (qsim_amd/fuser_learned.h) */
+};
+
+/* Parses qsim's circuit text format up to `maxtime` (CircuitQsimParser::
+ * FromStream, lib/circuit_qsim_parser.h:52-147).  NULL on a parse error. */
+void* qsim_driver_load(const char* text, unsigned maxtime);
+void qsim_driver_free(void* circuit);
+unsigned qsim_driver_num_qubits(void* circuit);
+unsigned qsim_driver_num_gates(void* circuit);
+
+/* Fuses the circuit into passes of at most max_fused_size qubits
+ * (MultiQubitGateFuser::FuseGates, lib/fuser_mqubit.h:185-526).  With
+ * matrix_prec == QSIM_HIP_F64 the fused matrices are the products of the
+ * FP64 gate matrices (lib/fuser.h:149-179 evaluated in double). */
+int qsim_driver_fuse(void* circuit, unsigned max_fused_size, int fuser,
+                     int matrix_prec);
+
+unsigned qsim_driver_num_passes(void* circuit);
+
+/* Pass i: its qubit count, qubits (ascending, up to 6) and gate count. */
+int qsim_driver_pass_info(void* circuit, unsigned i, unsigned* k,
+                          unsigned* qubits, unsigned* num_gates);
+
+/* Applies passes [first, first + count) to a state of the circuit's size
+ * (the apply loop of QSimRunner::Run, lib/run_qsim.h:264-280); measurement
+ * passes draw from mt19937(seed).  Asynchronous for gate passes. */
+int qsim_driver_run_passes(void* circuit, void* state, int prec,
+                           unsigned first, unsigned count, unsigned seed);
+
+/* All passes. */
+int qsim_driver_run(void* circuit, void* state, int prec, unsigned seed);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QSIM_HIP_DRIVER_H_ */

@@ -82,6 +82,8 @@ SIGNATURES = [
     ("qsim_hip_get_gate_method", _i, []),
     ("qsim_hip_event_record", _i, [_i]),
     ("qsim_hip_event_elapsed_ms", _i, [_i, _i, ctypes.POINTER(ctypes.c_float)]),
+    ("qsim_hip_pass_timer_start", _i, [_u]),
+    ("qsim_hip_pass_timer_stop", _i, [ctypes.POINTER(ctypes.c_float), _u, _u32p]),
 ]
 
 
@@ -138,6 +140,23 @@ def set_gate_method(method: int):
 
 def get_gate_method() -> int:
     return lib().qsim_hip_get_gate_method()
+
+
+class PassTimer:
+    """Kernel durations of every gate pass between start() and stop()."""
+
+    @staticmethod
+    def start(capacity: int = 65536):
+        _check(lib().qsim_hip_pass_timer_start(capacity))
+
+    @staticmethod
+    def stop(max_count: int = 65536) -> np.ndarray:
+        out = np.zeros(max_count, dtype=np.float32)
+        cnt = ctypes.c_uint32(0)
+        _check(lib().qsim_hip_pass_timer_stop(
+            out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), max_count,
+            ctypes.byref(cnt)))
+        return out[:min(cnt.value, max_count)].copy()
 
 
 class Timer:

@@ -43,6 +43,14 @@ Device g_dev;
 hipStream_t g_stream = nullptr;
 std::atomic<int> g_method{QSIM_HIP_METHOD_AUTO};
 
+struct PassTimer {
+  bool on = false;
+  unsigned capacity = 0;
+  unsigned count = 0;
+  std::vector<hipEvent_t> ev;
+};
+PassTimer g_timer;
+
 }  // namespace
 
 void SetError(const char* fmt, ...) {
@@ -132,6 +140,18 @@ void* MatrixBuffer(size_t bytes) {
   int slot = g_dev.mat_next;
   g_dev.mat_next = (slot + 1) % Device::kSlots;
   return g_dev.mat_dev + size_t(slot) * Device::kSlotBytes;
+}
+
+void PassTimerBegin() {
+  if (g_timer.on && g_timer.count < g_timer.capacity)
+    (void)hipEventRecord(g_timer.ev[2 * g_timer.count], g_stream);
+}
+
+void PassTimerEnd() {
+  if (g_timer.on && g_timer.count < g_timer.capacity) {
+    (void)hipEventRecord(g_timer.ev[2 * g_timer.count + 1], g_stream);
+    ++g_timer.count;
+  }
 }
 
 int StageMatrix(void* dev, const void* host, size_t bytes) {
@@ -557,6 +577,36 @@ int qsim_hip_event_elapsed_ms(int a, int b, float* ms) {
   }
   QHIP_CHECK(hipEventSynchronize(g_dev.ev[b]));
   QHIP_CHECK(hipEventElapsedTime(ms, g_dev.ev[a], g_dev.ev[b]));
+  return QSIM_HIP_OK;
+}
+
+int qsim_hip_pass_timer_start(unsigned capacity) {
+  QHIP_TRY(EnsureInit());
+  if (capacity == 0 || capacity > (1u << 20)) {
+    SetError("pass timer capacity %u out of range", capacity);
+    return QSIM_HIP_ERR_ARG;
+  }
+  while (g_timer.ev.size() < 2 * size_t(capacity)) {
+    hipEvent_t e;
+    QHIP_CHECK(hipEventCreate(&e));
+    g_timer.ev.push_back(e);
+  }
+  g_timer.capacity = capacity;
+  g_timer.count = 0;
+  g_timer.on = true;
+  return QSIM_HIP_OK;
+}
+
+int qsim_hip_pass_timer_stop(float* ms, unsigned max_count, unsigned* count) {
+  QHIP_TRY(EnsureInit());
+  g_timer.on = false;
+  QHIP_CHECK(hipStreamSynchronize(g_stream));
+  unsigned n = std::min(g_timer.count, max_count);
+  for (unsigned i = 0; i < n; ++i) {
+    QHIP_CHECK(hipEventElapsedTime(&ms[i], g_timer.ev[2 * i],
+                                   g_timer.ev[2 * i + 1]));
+  }
+  if (count) *count = g_timer.count;
   return QSIM_HIP_OK;
 }
 

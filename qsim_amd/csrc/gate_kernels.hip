@@ -247,8 +247,10 @@ template <typename FP, int K, typename M>
 int LaunchReg(typename Cplx<FP>::T* s, const GateGeom& g, const M& m) {
   uint64_t blocks = (g.ngroups + 255) / 256;
   blocks = std::min<uint64_t>(std::max<uint64_t>(blocks, 1), uint64_t{1} << 20);
+  PassTimerBegin();
   hipLaunchKernelGGL((GateRegKernel<FP, K, M>), dim3(unsigned(blocks)),
                      dim3(256), 0, Stream(), s, g, m);
+  PassTimerEnd();
   QHIP_LAUNCHED("GateRegKernel");
   return QSIM_HIP_OK;
 }
@@ -262,8 +264,10 @@ int LaunchTile(typename Cplx<FP>::T* s, const TileGeom& g, const M& m) {
     return QSIM_HIP_ERR_RUNTIME;
   }
   uint64_t blocks = std::min<uint64_t>(g.ntiles, uint64_t{1} << 16);
+  PassTimerBegin();
   hipLaunchKernelGGL((GateTileKernel<FP, K, M>), dim3(unsigned(blocks)),
                      dim3(256), lds, Stream(), s, g, m);
+  PassTimerEnd();
   QHIP_LAUNCHED("GateTileKernel");
   return QSIM_HIP_OK;
 }

@@ -52,6 +52,10 @@ int StageMatrix(void* dev, const void* host, size_t bytes);
 
 int NumCUs();
 
+// Per-pass timer hooks around gate kernel launches (no-ops unless started).
+void PassTimerBegin();
+void PassTimerEnd();
+
 // ---- complex element types -------------------------------------------------
 
 template <typename FP> struct Cplx;

@@ -1,0 +1,97 @@
+"""Whole-circuit runs through libqsim_hip_driver.so (include/qsim_hip_driver.h).
+
+The driver library is qsim's own host code -- CircuitQsimParser,
+MultiQubitGateFuser and the QSimRunner apply loop -- compiled over the gfx950
+backend's C++ adapter (include/qsim_amd/).  This module is its ctypes face.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import F32, F64, State, _check, lib
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DRIVER_PATH = os.path.join(HERE, "lib", "libqsim_hip_driver.so")
+
+FUSER_MQUBIT = 0
+FUSER_LEARNED = 1
+
+_dll = None
+
+
+def driver_lib():
+    global _dll
+    if _dll is None:
+        lib()  # loads libqsim_hip.so first (the driver links to it)
+        if not os.path.exists(DRIVER_PATH):
+            raise FileNotFoundError(
+                f"{DRIVER_PATH} is missing: it is built by "
+                "`python -m qsim_amd.build` where qsim's headers are present")
+        d = ctypes.CDLL(DRIVER_PATH)
+        u, i, vp = ctypes.c_uint, ctypes.c_int, ctypes.c_void_p
+        up = ctypes.POINTER(ctypes.c_uint32)
+        for name, res, args in (
+                ("qsim_driver_load", vp, [ctypes.c_char_p, u]),
+                ("qsim_driver_free", None, [vp]),
+                ("qsim_driver_num_qubits", u, [vp]),
+                ("qsim_driver_num_gates", u, [vp]),
+                ("qsim_driver_fuse", i, [vp, u, i, i]),
+                ("qsim_driver_num_passes", u, [vp]),
+                ("qsim_driver_pass_info", i, [vp, u, up, up, up]),
+                ("qsim_driver_run_passes", i, [vp, vp, i, u, u, u]),
+                ("qsim_driver_run", i, [vp, vp, i, u])):
+            f = getattr(d, name)
+            f.restype = res
+            f.argtypes = args
+        _dll = d
+    return _dll
+
+
+class Circuit:
+    """A qsim text circuit parsed, fused and run by qsim's host code."""
+
+    def __init__(self, text: str, maxtime: int = 0xFFFFFFFF):
+        self.h = driver_lib().qsim_driver_load(text.encode(), maxtime)
+        if not self.h:
+            raise ValueError("qsim's parser rejected the circuit")
+        self.num_qubits = driver_lib().qsim_driver_num_qubits(self.h)
+        self.num_gates = driver_lib().qsim_driver_num_gates(self.h)
+
+    def fuse(self, max_fused_size: int, fuser: int = FUSER_MQUBIT,
+             f64_matrices: bool = False):
+        _check(driver_lib().qsim_driver_fuse(self.h, max_fused_size, fuser,
+                                             F64 if f64_matrices else F32))
+        return self.num_passes
+
+    @property
+    def num_passes(self) -> int:
+        return driver_lib().qsim_driver_num_passes(self.h)
+
+    def pass_info(self, i: int):
+        k = ctypes.c_uint32(0)
+        ng = ctypes.c_uint32(0)
+        qs = (ctypes.c_uint32 * 64)()
+        _check(driver_lib().qsim_driver_pass_info(
+            self.h, i, ctypes.byref(k), qs, ctypes.byref(ng)))
+        return list(qs[:k.value]), ng.value
+
+    def passes(self):
+        return [self.pass_info(i) for i in range(self.num_passes)]
+
+    def run(self, state: State, seed: int = 1, first: int = 0,
+            count: int = None):
+        if count is None:
+            count = self.num_passes - first
+        _check(driver_lib().qsim_driver_run_passes(
+            self.h, state.ptr, state.prec, first, count, seed))
+
+    def __del__(self):
+        try:
+            if self.h:
+                driver_lib().qsim_driver_free(self.h)
+        except Exception:
+            pass

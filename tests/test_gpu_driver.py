@@ -1,0 +1,101 @@
+"""Whole circuits through qsim's host code (parser, MultiQubitGateFuser,
+apply loop) on the gfx950 backend, against the CPU oracle."""
+
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+import qsim_amd
+from helpers import from_interleaved, reference_runs, rel_l2
+from qsim_amd import circuits, driver
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def grid20():
+    text = circuits.grid_circuit(20, 14, seed=0)
+    s32, _ = oracle.Circuit(text, np.float32).run()
+    s64, _ = oracle.Circuit(text, np.float64).run()
+    return text, s32, s64
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5, 6])
+def test_grid20_fused_vs_oracle(grid20, k):
+    text, s32, s64 = grid20
+    c = driver.Circuit(text)
+    c.fuse(k)
+    ss = qsim_amd.StateSpace(np.float32)
+    st = ss.create(20)
+    ss.set_state_zero(st)
+    c.run(st)
+    got = st.to_numpy()
+    assert rel_l2(got, s32) < 1e-5
+    fid = abs(np.vdot(from_interleaved(s64), from_interleaved(got))) ** 2
+    print(f"k={k}: {c.num_passes} passes, rel-L2 vs oracle "
+          f"{rel_l2(got, s32):.2e}, fidelity vs FP64 {1 - fid:.2e} below 1",
+          flush=True)
+    assert fid > 1 - 1e-6
+    # FP64 fused matrices on the complex64 state
+    c.fuse(k, f64_matrices=True)
+    ss.set_state_zero(st)
+    c.run(st)
+    assert rel_l2(st.to_numpy(), s64) < 1e-5
+
+
+def test_grid20_complex128(grid20):
+    text, _, s64 = grid20
+    c = driver.Circuit(text)
+    c.fuse(4)
+    ss = qsim_amd.StateSpace(np.float64)
+    st = ss.create(20)
+    ss.set_state_zero(st)
+    c.run(st)
+    assert rel_l2(st.to_numpy(), s64) < 1e-12
+
+
+def test_grid20_matches_reference_fixture(grid20):
+    text, s32, _ = grid20
+    g = reference_runs()["grid20"]
+    c = driver.Circuit(text)
+    c.fuse(2)
+    ss = qsim_amd.StateSpace(np.float32)
+    st = ss.create(20)
+    ss.set_state_zero(st)
+    c.run(st)
+    got = from_interleaved(st.to_numpy())
+    for i, (re, im) in g["avx"]["amplitudes"].items():
+        assert abs(got[int(i)] - complex(re, im)) < 1e-6
+    assert abs(ss.norm(st) - g["avx"]["norm"]) < 1e-6
+
+
+def test_random_all_gate_circuits(grid20):
+    # Every gate name of the format, controls included, fused up to k=4.
+    for case in reference_runs()["random10"]:
+        c = driver.Circuit(case["circuit"])
+        c.fuse(4)
+        ref = np.array(case["state"])
+        ref = ref[:, 0] + 1j * ref[:, 1]
+        for dtype, tol in ((np.float32, 1e-5), (np.float64, 1e-12)):
+            ss = qsim_amd.StateSpace(dtype)
+            st = ss.create(c.num_qubits)
+            ss.set_state_zero(st)
+            c.run(st)
+            assert rel_l2(from_interleaved(st.to_numpy()), ref) < tol
+
+
+def test_measurement_circuit_matches_oracle():
+    # Mid-circuit measurements: same mt19937 draws, same collapsed state.
+    text = ("6\n0 h 0\n0 h 1\n0 h 2\n0 h 3\n0 h 4\n0 h 5\n1 cz 0 1\n"
+            "1 cz 2 3\n2 m 0 2 4\n3 fs 1 5 0.4 0.2\n4 rx 3 0.3\n5 m 1 3 5\n")
+    for seed in (1, 2, 3, 7):
+        s_or, mres = oracle.Circuit(text, np.float64).run(seed=seed)
+        c = driver.Circuit(text)
+        c.fuse(2)
+        ss = qsim_amd.StateSpace(np.float64)
+        st = ss.create(6)
+        ss.set_state_zero(st)
+        c.run(st, seed=seed)
+        assert rel_l2(st.to_numpy(), s_or) < 1e-12, seed

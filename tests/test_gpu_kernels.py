@@ -346,10 +346,17 @@ def test_grid20_circuit_vs_oracle_and_reference(ss32, sim):
     cplx = from_interleaved(s)
     for i, (re, im) in amps.items():
         assert abs(cplx[int(i)] - complex(re, im)) < 1e-6
-    # Samples of the reference on its own state: at most a rare boundary flip.
+    # Identical draws on the identical state: identical bitstrings.
     bits = ss32.sample(st, 1000, 0)
+    np.testing.assert_array_equal(bits, oracle.sample(20, s, 1000, 0))
+    # Against the reference's own, independently evolved FP32 state the
+    # cumulative sums differ by ~1e-9, which moves a few of 1000 draws across
+    # an amplitude boundary (each such flip lands on a neighbouring index).
     ref_bits = np.array(g["samples_seed0"], dtype=np.uint64)
-    assert np.count_nonzero(bits != ref_bits) <= 2
+    flips = np.flatnonzero(bits != ref_bits)
+    print(f"grid20: {len(flips)} of 1000 samples differ from the reference's "
+          f"independently evolved state", flush=True)
+    assert len(flips) <= 10
 
 
 def test_large_state_round_trip(sim):
