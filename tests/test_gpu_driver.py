@@ -33,16 +33,26 @@ def test_grid20_fused_vs_oracle(grid20, k):
     c.run(st)
     got = st.to_numpy()
     assert rel_l2(got, s32) < 1e-5
-    fid = abs(np.vdot(from_interleaved(s64), from_interleaved(got))) ** 2
+    # qsim's FP32 gate matrices are unitary only to ~3e-8 (sqrt(W) holds
+    # float(1/sqrt 2)), so the FP32-matrix state loses norm: judge its
+    # direction by the normalised fidelity.
+    g, f = from_interleaved(got), from_interleaved(s64)
+    fid_raw = abs(np.vdot(f, g)) ** 2
+    fid_norm = fid_raw / (np.vdot(g, g).real * np.vdot(f, f).real)
     print(f"k={k}: {c.num_passes} passes, rel-L2 vs oracle "
-          f"{rel_l2(got, s32):.2e}, fidelity vs FP64 {1 - fid:.2e} below 1",
-          flush=True)
-    assert fid > 1 - 1e-6
-    # FP64 fused matrices on the complex64 state
+          f"{rel_l2(got, s32):.2e}, 1-F vs FP64 {1 - fid_raw:.2e} "
+          f"(normalised {1 - fid_norm:.2e})", flush=True)
+    assert fid_norm > 1 - 1e-9
+    # FP64 fused matrices (built on the host in double) on the complex64
+    # state meet the unnormalised bar F >= 1 - 1e-6.
     c.fuse(k, f64_matrices=True)
     ss.set_state_zero(st)
     c.run(st)
-    assert rel_l2(st.to_numpy(), s64) < 1e-5
+    got64m = st.to_numpy()
+    assert rel_l2(got64m, s64) < 1e-5
+    fid = abs(np.vdot(f, from_interleaved(got64m))) ** 2
+    print(f"k={k} FP64 matrices: 1-F vs FP64 {1 - fid:.2e}", flush=True)
+    assert fid > 1 - 1e-6
 
 
 def test_grid20_complex128(grid20):
