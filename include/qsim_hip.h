@@ -36,10 +36,14 @@ enum { QSIM_HIP_F32 = 0, QSIM_HIP_F64 = 1 };
 
 /* Arithmetic used for an uncontrolled gate pass (qsim_hip_set_gate_method). */
 enum {
-  QSIM_HIP_METHOD_AUTO = 0,      /* FP32/FP64 vector kernels */
-  QSIM_HIP_METHOD_VECTOR = 1,    /* always the vector kernels */
-  QSIM_HIP_METHOD_MFMA_BF16X3 = 2,  /* k>=5: split-BF16 x3 MFMA, FP32 accum */
-  QSIM_HIP_METHOD_MFMA_FP16X2 = 3   /* k>=5: scaled two-term FP16 MFMA */
+  /* complex64, uncontrolled k >= 4: FP32 MFMA (exact FP32 products and
+   * sums); everything else: the FP32 / FP64 vector kernels */
+  QSIM_HIP_METHOD_AUTO = 0,
+  QSIM_HIP_METHOD_VECTOR = 1,       /* always the vector kernels */
+  QSIM_HIP_METHOD_MFMA_BF16X3 = 2,  /* as AUTO, but k >= 5 on split-BF16 x3
+                                       MFMA with FP32 accumulation */
+  QSIM_HIP_METHOD_MFMA_FP16X2 = 3   /* reserved: scaled two-term FP16 MFMA
+                                       (selecting it is rejected) */
 };
 
 /* ---- context ----------------------------------------------------------- */

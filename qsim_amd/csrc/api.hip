@@ -469,9 +469,8 @@ int qsim_hip_apply_controlled_gate(int prec, void* s, unsigned n,
   }
   const size_t melems = size_t{2} << (2 * k);
   int method = g_method.load();
-  if (prec == QSIM_HIP_F32 && nc == 0 && k >= 5 &&
-      (method == QSIM_HIP_METHOD_MFMA_BF16X3 ||
-       method == QSIM_HIP_METHOD_MFMA_FP16X2)) {
+  if (prec == QSIM_HIP_F32 && nc == 0 && k >= 4 &&
+      method != QSIM_HIP_METHOD_VECTOR) {
     std::vector<double> m64(melems);
     if (matrix_prec == QSIM_HIP_F64) {
       std::memcpy(m64.data(), matrix, melems * sizeof(double));
@@ -552,6 +551,10 @@ int qsim_hip_set_gate_method(int method) {
   if (method < QSIM_HIP_METHOD_AUTO || method > QSIM_HIP_METHOD_MFMA_FP16X2) {
     SetError("unknown gate method %d", method);
     return QSIM_HIP_ERR_ARG;
+  }
+  if (method == QSIM_HIP_METHOD_MFMA_FP16X2) {
+    SetError("the scaled FP16 MFMA method is not built");
+    return QSIM_HIP_ERR_UNSUPPORTED;
   }
   g_method.store(method);
   return QSIM_HIP_OK;

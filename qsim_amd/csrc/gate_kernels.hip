@@ -25,6 +25,7 @@
 
 #include "internal.h"
 #include "reduce.h"
+#include "tile.h"
 
 namespace qhip {
 
@@ -41,6 +42,14 @@ struct MatPtr {
   const FP* __restrict__ p;
   __device__ __forceinline__ FP operator()(int i) const { return p[i]; }
 };
+
+// Largest tile per element type: 64 KiB of LDS (2^13 complex64 or 2^12
+// complex128).  Passes with k <= 5 use half of it unless the larger tile
+// splits into fewer contiguous runs; k = 6 always uses it.
+template <typename FP, int K>
+constexpr unsigned TileBitsMax() {
+  return sizeof(FP) == 4 ? 13u : 12u;
+}
 
 template <int K>
 __device__ __forceinline__ uint64_t MemberOffset(unsigned j,
@@ -125,18 +134,12 @@ GateTileKernel(typename Cplx<FP>::T* __restrict__ s, TileGeom g, M m) {
   extern __shared__ char lds_raw[];
   C* lds = reinterpret_cast<C*>(lds_raw);
   const unsigned T = 1u << g.tb;
-  const unsigned lowmask = (1u << g.Lc) - 1;
   const unsigned ngroups = T >> K;
+  constexpr int PER = (size_t{1} << TileBitsMax<FP, K>()) / 256;
   for (uint64_t tile = blockIdx.x; tile < g.ntiles; tile += gridDim.x) {
     const uint64_t base =
         InsertZeros(tile << g.Lc, g.outer, g.nouter) | g.cbits_high;
-#pragma unroll 4
-    for (unsigned a = threadIdx.x; a < T; a += 256) {
-      uint64_t gi = base + (a & lowmask);
-      const unsigned hi = a >> g.Lc;
-      for (unsigned b = 0; b < g.h; ++b) gi += ((hi >> b) & 1u) ? g.hbit[b] : 0;
-      lds[a] = s[gi];
-    }
+    TileLoad<PER>(s, base, g, lds);
     __syncthreads();
     for (unsigned gr = threadIdx.x; gr < ngroups; gr += 256) {
       unsigned tb0 = gr;
@@ -148,19 +151,14 @@ GateTileKernel(typename Cplx<FP>::T* __restrict__ s, TileGeom g, M m) {
       if ((tb0 & g.low_cmask) != g.low_cvals) continue;
       C v[D];
 #pragma unroll
-      for (int j = 0; j < D; ++j) v[j] = lds[tb0 + TileMemberOffset<K>(j, g.tpos)];
+      for (int j = 0; j < D; ++j)
+        v[j] = lds[TileSwz(tb0 + TileMemberOffset<K>(j, g.tpos))];
       MatVec<FP, K>(v, m, [&](int r, C val) {
-        lds[tb0 + TileMemberOffset<K>(r, g.tpos)] = val;
+        lds[TileSwz(tb0 + TileMemberOffset<K>(r, g.tpos))] = val;
       });
     }
     __syncthreads();
-#pragma unroll 4
-    for (unsigned a = threadIdx.x; a < T; a += 256) {
-      uint64_t gi = base + (a & lowmask);
-      const unsigned hi = a >> g.Lc;
-      for (unsigned b = 0; b < g.h; ++b) gi += ((hi >> b) & 1u) ? g.hbit[b] : 0;
-      s[gi] = lds[a];
-    }
+    TileStore<PER>(s, base, g, lds);
     __syncthreads();
   }
 }
@@ -235,13 +233,7 @@ int CheckQubits(unsigned n, const unsigned* qs, unsigned k, const unsigned* cqs,
 template <typename FP, int K>
 constexpr bool kByValue = sizeof(FP) * (2u << (2 * K)) <= 2048;
 
-// Tile bits per element type and gate size: 32 KiB of LDS for k <= 5,
-// 64 KiB for k = 6 (at least 2^7 groups per 256-thread workgroup).
-template <typename FP>
-unsigned TileBits(unsigned k) {
-  unsigned base = sizeof(FP) == 4 ? 12 : 11;
-  return k <= 5 ? base : base + 1;
-}
+
 
 template <typename FP, int K, typename M>
 int LaunchReg(typename Cplx<FP>::T* s, const GateGeom& g, const M& m) {
@@ -263,7 +255,7 @@ int LaunchTile(typename Cplx<FP>::T* s, const TileGeom& g, const M& m) {
     SetError("tile of %zu bytes exceeds the 64 KiB LDS budget", lds);
     return QSIM_HIP_ERR_RUNTIME;
   }
-  uint64_t blocks = std::min<uint64_t>(g.ntiles, uint64_t{1} << 16);
+  uint64_t blocks = std::min<uint64_t>(g.ntiles, uint64_t{1} << 20);
   PassTimerBegin();
   hipLaunchKernelGGL((GateTileKernel<FP, K, M>), dim3(unsigned(blocks)),
                      dim3(256), lds, Stream(), s, g, m);
@@ -311,31 +303,45 @@ int DispatchK(typename Cplx<FP>::T* s, unsigned n, const unsigned* qs,
   }
 
   // Tile geometry: find tb <= n, h with h = #targets >= tb - h, and enough
-  // outer bits left for the high controls.
+  // outer bits left for the high controls.  The tile is 2^h contiguous runs;
+  // the larger tile (64 KiB) is taken when it leaves fewer runs, or for k = 6.
   TileGeom g{};
-  unsigned TB = std::min(TileBits<FP>(K), n);
-  for (;;) {
-    unsigned h = 0, Lc = TB;
-    for (unsigned hh = 0; hh <= unsigned(K); ++hh) {
-      if (hh > TB) break;
-      unsigned lc = TB - hh;
-      unsigned cnt = 0;
-      for (int b = 0; b < K; ++b) cnt += qs[b] >= lc;
-      if (cnt == hh) {
-        h = hh;
-        Lc = lc;
-        break;
+  auto fit = [&](unsigned TB, unsigned* hout, unsigned* lcout,
+                 unsigned* tbout) {
+    for (;;) {
+      unsigned h = 0, Lc = TB;
+      for (unsigned hh = 0; hh <= unsigned(K); ++hh) {
+        if (hh > TB) break;
+        unsigned lc = TB - hh;
+        unsigned cnt = 0;
+        for (int b = 0; b < K; ++b) cnt += qs[b] >= lc;
+        if (cnt == hh) {
+          h = hh;
+          Lc = lc;
+          break;
+        }
       }
+      unsigned hc = 0;
+      for (unsigned i = 0; i < nc; ++i) hc += cqs[i] >= Lc;
+      if (n >= TB + hc) {
+        *hout = h;
+        *lcout = Lc;
+        *tbout = TB;
+        return;
+      }
+      --TB;
     }
-    unsigned hc = 0;
-    for (unsigned i = 0; i < nc; ++i) hc += cqs[i] >= Lc;
-    if (n >= TB + hc) {
-      g.Lc = Lc;
-      g.h = h;
-      g.tb = TB;
-      break;
+  };
+  const unsigned small_tb = TileBitsMax<FP, K>() - (K >= 6 ? 0u : 1u);
+  fit(std::min(small_tb, n), &g.h, &g.Lc, &g.tb);
+  if (K < 6 && n > small_tb) {
+    unsigned h2, lc2, tb2;
+    fit(std::min(TileBitsMax<FP, K>(), n), &h2, &lc2, &tb2);
+    if (h2 < g.h) {
+      g.h = h2;
+      g.Lc = lc2;
+      g.tb = tb2;
     }
-    --TB;
   }
   std::vector<unsigned> outer;
   unsigned hi = 0;

@@ -396,3 +396,54 @@ def test_error_paths(ss32, sim):
         sim.apply_controlled_gate([1], [1], 1, np.zeros(8, np.float32), st)
     # Allocation beyond the device -> Null (None), not an exception.
     assert qsim_amd.StateSpace(np.float64).create(40) is None
+
+
+@pytest.mark.parametrize("method", [qsim_amd.METHOD_VECTOR, qsim_amd.METHOD_AUTO])
+def test_fp32_methods_vs_oracle(method, sim):
+    # FP32 vector kernels and FP32 MFMA (k >= 4) meet the same 1e-6 bar, for
+    # low, high, mixed and random targets.
+    rng = np.random.default_rng(21)
+    qsim_amd.set_gate_method(method)
+    try:
+        for n in (9, 10, 11, 12, 13, 16, 19):
+            for k in (4, 5, 6):
+                if k + 5 > n:
+                    continue
+                for qs in _target_sets(n, k, rng):
+                    u = random_matrix(k, rng)
+                    m = to_interleaved(u, np.float32)
+                    s = random_state(n, rng, np.float32)
+                    st = upload(s, n, np.float32)
+                    sim.apply_gate(qs, m, st)
+                    oracle.apply_gate(n, s, qs, m)
+                    err = rel_l2(st.to_numpy(), s)
+                    assert err < 1e-6, (method, n, k, qs, err)
+    finally:
+        qsim_amd.set_gate_method(qsim_amd.METHOD_AUTO)
+
+
+def test_bf16x3_mfma_accuracy(sim):
+    # Split-BF16 x3 products carry ~2^-16 relative error per product: per pass
+    # the state stays within 5e-5 relative L2 of the FP64 result.
+    rng = np.random.default_rng(22)
+    qsim_amd.set_gate_method(qsim_amd.METHOD_MFMA_BF16X3)
+    try:
+        for n in (11, 12, 16, 20):
+            for k in (5, 6):
+                for qs in _target_sets(n, k, rng):
+                    u = random_matrix(k, rng)
+                    s = random_state(n, rng, np.float64)
+                    st = upload(s.astype(np.float32), n, np.float32)
+                    sim.apply_gate(qs, to_interleaved(u, np.float64), st)
+                    oracle.apply_gate(n, s, qs, to_interleaved(u, np.float64))
+                    err = rel_l2(st.to_numpy(), s)
+                    assert err < 5e-5, (n, k, qs, err)
+    finally:
+        qsim_amd.set_gate_method(qsim_amd.METHOD_AUTO)
+
+
+def test_fp16_method_is_rejected():
+    with pytest.raises(qsim_amd.QsimHipError) as e:
+        qsim_amd.set_gate_method(qsim_amd.METHOD_MFMA_FP16X2)
+    assert e.value.code == -4
+    assert qsim_amd.get_gate_method() == qsim_amd.METHOD_AUTO
