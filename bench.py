@@ -3,16 +3,27 @@
 One step = one full circuit (|0..0> preparation plus every fused pass) on a
 complex64 state resident in HBM.  The circuit is the 4x8 grid, 20 cycles,
 seed 0 construction of qsim_amd/circuits.py, fused by qsim's
-MultiQubitGateFuser (max_fused_size 4) and applied by qsim's own apply loop
-over the gfx950 kernels.
+MultiQubitGateFuser and applied by qsim's own apply loop over the gfx950
+kernels.  Three paths run on the same circuit:
 
-Prints one JSON line: seconds per circuit (value), gate passes/s, the
-roofline of the dominant gate-pass kernel from HIP events recorded around
-every pass inside the timed region, and qsim's AVX/OpenMP CPU path timed on a
-bounded sample of the same circuit on this host.
+  plain-fp32-vector  max_fused_size 4, FP32 vector kernels (qsim's method)
+  fp32-mfma          max_fused_size 4, exact-FP32 MFMA for the k=4 passes
+  bf16x3-k6          max_fused_size 6, fused matrices built in FP64 on the
+                     host, split-BF16x3 MFMA for k>=5 passes
+
+Each path's fidelity 1-|<psi64|psi>|^2 is measured on the same construction
+at 28 qubits against a complex128 run of the same circuit on the GPU (FP64
+gate matrices and FP64 vector kernels; that path is checked against the CPU
+FP64 oracle by the tests).  `value` is the fastest path that meets
+F >= 1 - 1e-6.
+
+Prints one JSON line with the roofline of that path's dominant gate kernel
+(HIP events around every pass inside the timed region) and qsim's own
+AVX/OpenMP CPU path timed on a bounded sample of the same circuit.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--qubits 32]
-                    [--cycles 20] [--max-fused 4] [--cpu-passes P | --no-cpu]
+                    [--cycles 20] [--paths a,b,c] [--cpu-passes P | --no-cpu]
+                    [--no-fidelity]
 """
 
 from __future__ import annotations
@@ -20,7 +31,6 @@ from __future__ import annotations
 import argparse
 import json
 import os
-import subprocess
 import sys
 import tempfile
 import time
@@ -34,74 +44,86 @@ HBM_PEAK_GBPS = 8000.0
 METRIC = ("32q circuit time + HBM GB/s, learned/MFMA vs plain FP32 fusion "
           "at F>=1-1e-6")
 
+# name -> (max_fused_size, fused matrices in FP64, gate method)
+PATHS = {
+    "plain-fp32-vector": (4, False, 1),
+    "fp32-mfma": (4, False, 0),
+    "bf16x3-k6": (6, True, 2),
+}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def kernel_family(qubits):
-    """Which gate kernel a pass runs: the register path when every target is
-    at qubit >= 5, the LDS tile path otherwise (qsim_amd/csrc/gate_kernels.hip)."""
-    return "GateRegKernel" if (not qubits or min(qubits) >= 5) else "GateTileKernel"
+def kernel_family(qubits, method, n):
+    """The gate kernel a pass runs (qsim_amd/csrc: gate_kernels.hip and
+    mfma_kernels.hip dispatch)."""
+    k = len(qubits)
+    low = bool(qubits) and min(qubits) < 5
+    if method != 1 and 4 <= k <= 6 and n >= k + 5:
+        if method == 2 and k >= 5:
+            return "GateBf16TileKernel" if low else "GateBf16RegKernel"
+        return "GateMfmaTileKernel" if low else "GateMfmaRegKernel"
+    return "GateTileKernel" if low else "GateRegKernel"
 
 
-def run_gpu(args, text):
+def run_path(args, text, name):
     import qsim_amd
     from qsim_amd import driver
 
-    name, mem, cus = qsim_amd.device_info()
-    log(f"device: {name}, {mem / 2**30:.0f} GiB, {cus} CUs")
+    kmax, f64m, method = PATHS[name]
     circ = driver.Circuit(text)
     t0 = time.time()
-    npass = circ.fuse(args.max_fused, driver.FUSER_MQUBIT)
+    npass = circ.fuse(kmax, driver.FUSER_MQUBIT, f64_matrices=f64m)
     fuse_s = time.time() - t0
     passes = circ.passes()
-    ks = [len(q) for q, _ in passes]
-    log(f"circuit: {circ.num_qubits} qubits, {circ.num_gates} gates -> "
-        f"{npass} fused passes (k histogram {np.bincount(ks).tolist()}), "
+    n = circ.num_qubits
+    log(f"[{name}] {circ.num_gates} gates -> {npass} passes "
+        f"(k histogram {np.bincount([len(q) for q, _ in passes]).tolist()}), "
         f"fuse {fuse_s:.3f}s")
-
+    qsim_amd.set_gate_method(method)
     ss = qsim_amd.StateSpace(np.float32)
-    st = ss.create(circ.num_qubits)
+    st = ss.create(n)
     if st is None:
         raise RuntimeError("cannot allocate the state")
-    for w in range(args.warmup):
-        ss.set_state_zero(st)
-        circ.run(st)
+    try:
+        for w in range(args.warmup):
+            ss.set_state_zero(st)
+            circ.run(st)
+            qsim_amd.synchronize()
         qsim_amd.synchronize()
-        log(f"warmup {w + 1}/{args.warmup} done")
+        qsim_amd.PassTimer.start(npass * args.steps + 16)
+        qsim_amd.Timer.record(0)
+        t_wall = time.time()
+        for _ in range(args.steps):
+            ss.set_state_zero(st)
+            circ.run(st)
+        qsim_amd.Timer.record(1)
+        qsim_amd.synchronize()
+        wall = time.time() - t_wall
+        total_ms = qsim_amd.Timer.elapsed_ms(0, 1)
+        pass_ms = qsim_amd.PassTimer.stop(npass * args.steps + 16)
+        norm = ss.norm(st)
+    finally:
+        qsim_amd.set_gate_method(0)
+        st.free()
+    log(f"[{name}] {args.steps} circuits in {total_ms:.1f} ms "
+        f"(wall {wall:.3f}s), {len(pass_ms)} passes timed, norm {norm:.9f}")
 
-    qsim_amd.synchronize()
-    qsim_amd.PassTimer.start(npass * args.steps + 16)
-    qsim_amd.Timer.record(0)
-    t_wall = time.time()
-    for _ in range(args.steps):
-        ss.set_state_zero(st)
-        circ.run(st)
-    qsim_amd.Timer.record(1)
-    qsim_amd.synchronize()
-    wall = time.time() - t_wall
-    total_ms = qsim_amd.Timer.elapsed_ms(0, 1)
-    pass_ms = qsim_amd.PassTimer.stop(npass * args.steps + 16)
-    log(f"timed: {args.steps} circuits in {total_ms:.1f} ms (wall {wall:.3f}s), "
-        f"{len(pass_ms)} passes timed")
-    norm = ss.norm(st)
-    log(f"final norm {norm:.9f}")
-
-    n = circ.num_qubits
-    bytes_per_pass = 16.0 * (1 << n)   # read + write every complex64 once
-    fam = [kernel_family(q) for q, _ in passes] * args.steps
+    bytes_per_pass = 16.0 * (1 << n)   # every complex64 read and written once
+    fams = [kernel_family(q, method, n) for q, _ in passes] * args.steps
     per_fam = {}
-    for f, ms in zip(fam, pass_ms):
+    for f, ms in zip(fams, pass_ms):
         per_fam.setdefault(f, []).append(float(ms))
     fam_stats = {}
     for f, v in per_fam.items():
         avg = float(np.mean(v))
-        fam_stats[f] = {"passes": len(v), "avg_ms": avg, "total_ms": float(np.sum(v)),
+        fam_stats[f] = {"passes": len(v), "avg_ms": avg,
+                        "total_ms": float(np.sum(v)),
                         "achieved_gbps": bytes_per_pass / (avg * 1e-3) / 1e9}
     dominant = max(fam_stats, key=lambda f: fam_stats[f]["total_ms"])
     d = fam_stats[dominant]
-    all_avg = float(np.mean(pass_ms))
     per_k = {}
     for (q, _), ms in zip(passes * args.steps, pass_ms):
         per_k.setdefault(len(q), []).append(float(ms))
@@ -109,29 +131,74 @@ def run_gpu(args, text):
                         "gbps": bytes_per_pass / (float(np.mean(v)) * 1e-3) / 1e9}
                for k, v in sorted(per_k.items())}
     return {
+        "name": name, "max_fused_size": kmax, "fp64_matrices": f64m,
         "num_qubits": n, "num_gates": circ.num_gates, "passes": npass,
-        "circuit_ms": total_ms / args.steps, "wall_s": wall, "norm": norm,
-        "fuse_s": fuse_s, "pass_ms_sum": float(np.sum(pass_ms)) / args.steps,
-        "dominant": dominant, "families": fam_stats, "per_k": k_stats,
-        "all_pass_avg_ms": all_avg,
+        "circuit_s": total_ms / args.steps / 1e3, "wall_s": wall,
+        "final_norm": norm, "fuse_s": fuse_s,
+        "pass_ms_sum": float(np.sum(pass_ms)) / args.steps,
+        "hbm_gbps_avg_pass": bytes_per_pass / (float(np.mean(pass_ms)) * 1e-3) / 1e9,
+        "gate_passes_per_s": npass / (total_ms / args.steps / 1e3),
+        "kernels": fam_stats, "per_k": k_stats,
         "roofline": {
             "bound": "hbm", "kernel": dominant,
             "achieved": d["achieved_gbps"], "peak": HBM_PEAK_GBPS,
             "unit": "GB/s", "frac": d["achieved_gbps"] / HBM_PEAK_GBPS,
-            "traffic": None,
-            "bytes_per_launch": bytes_per_pass,
+            "traffic": None, "bytes_per_launch": bytes_per_pass,
             "avg_launch_ms": d["avg_ms"],
         },
     }
 
 
-def run_cpu_baseline(args, circuit_file, gpu_passes):
+def fidelities(args, names):
+    """1 - |<psi64|psi>|^2 of each path at --fidelity-qubits qubits."""
+    import qsim_amd
+    from qsim_amd import circuits, driver
+
+    nq = args.fidelity_qubits
+    text = circuits.grid_circuit(nq, args.cycles, args.seed)
+    ss64 = qsim_amd.StateSpace(np.float64)
+    ref = ss64.create(nq)
+    ss64.set_state_zero(ref)
+    c64 = driver.Circuit(text)
+    c64.fuse(4, driver.FUSER_MQUBIT, f64_matrices=True)
+    c64.run(ref)
+    out = {}
+    ss32 = qsim_amd.StateSpace(np.float32)
+    tmp = ss64.create(nq)
+    for name in names:
+        kmax, f64m, method = PATHS[name]
+        c = driver.Circuit(text)
+        c.fuse(kmax, driver.FUSER_MQUBIT, f64_matrices=f64m)
+        qsim_amd.set_gate_method(method)
+        st = ss32.create(nq)
+        try:
+            ss32.set_state_zero(st)
+            c.run(st)
+            ss64.copy(st, tmp)       # complex64 -> complex128 on the device
+        finally:
+            qsim_amd.set_gate_method(0)
+            st.free()
+        ip = ss64.inner_product(ref, tmp)
+        n64 = ss64.norm(ref)
+        n32 = ss64.norm(tmp)
+        f_raw = abs(ip) ** 2
+        out[name] = {"qubits": nq, "one_minus_F": 1.0 - f_raw,
+                     "one_minus_F_normalised": 1.0 - f_raw / (n64 * n32),
+                     "norm": n32}
+        log(f"[{name}] {nq}q fidelity: 1-F = {1.0 - f_raw:.3e} "
+            f"(normalised {1.0 - f_raw / (n64 * n32):.3e})")
+    ref.free()
+    tmp.free()
+    return out
+
+
+def run_cpu_baseline(args, circuit_file):
     import oracle
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)
     p = args.cpu_passes
     log(f"cpu baseline: qsim SimulatorAVX, {threads} threads, first {p} passes")
     t0 = time.time()
-    info = oracle.run_reference(circuit_file, mode="avx", max_fused=args.max_fused,
+    info = oracle.run_reference(circuit_file, mode="avx", max_fused=4,
                                 threads=threads, passes=p, timeout=1800)
     wall = time.time() - t0
     per_pass = info["sim_s"] / max(1, min(p, info["fused_ops"]))
@@ -143,7 +210,7 @@ def run_cpu_baseline(args, circuit_file, gpu_passes):
         "value": per_circuit, "unit": "s/circuit", "cores": threads,
         "kind": "reference",
         "sample": (f"first {p} of {info['fused_ops']} fused passes "
-                   f"(MultiQubitGateFuser k={args.max_fused}) of the same "
+                   f"(MultiQubitGateFuser k=4) of the same "
                    f"{info['num_qubits']}q circuit on qsim's SimulatorAVX "
                    f"(oracle/_ref), {threads} OpenMP threads; seconds per "
                    f"pass x passes per circuit"),
@@ -159,7 +226,9 @@ def main():
     ap.add_argument("--qubits", type=int, default=32)
     ap.add_argument("--cycles", type=int, default=20)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--max-fused", type=int, default=4)
+    ap.add_argument("--paths", default=",".join(PATHS))
+    ap.add_argument("--fidelity-qubits", type=int, default=28)
+    ap.add_argument("--no-fidelity", action="store_true")
     ap.add_argument("--cpu-passes", type=int, default=6)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -167,15 +236,30 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not "
-              "supported (the state-vector shards by its top qubits across GPUs "
-              "only in design; see the sharding notes).", file=sys.stderr)
+              "supported (sharding across GPUs is a design note only).",
+              file=sys.stderr)
         sys.exit(2)
 
+    import qsim_amd
     from qsim_amd import circuits
+    name, mem, cus = qsim_amd.device_info()
+    log(f"device: {name}, {mem / 2**30:.0f} GiB, {cus} CUs")
     text = circuits.grid_circuit(args.qubits, args.cycles, args.seed)
     rows, cols = circuits.grid_shape(args.qubits)
+    names = [p for p in args.paths.split(",") if p]
+    results = {p: run_path(args, text, p) for p in names}
 
-    gpu = run_gpu(args, text)
+    fid = {}
+    if not args.no_fidelity:
+        fid = fidelities(args, names)
+        for p in names:
+            results[p]["fidelity"] = fid[p]
+
+    def ok(p):
+        return not fid or fid[p]["one_minus_F"] <= 1e-6
+    eligible = [p for p in names if ok(p)] or names
+    best = min(eligible, key=lambda p: results[p]["circuit_s"])
+    b = results[best]
 
     cpu = None
     if not args.no_cpu:
@@ -184,20 +268,20 @@ def main():
             with open(cf, "w") as f:
                 f.write(text)
             try:
-                cpu = run_cpu_baseline(args, cf, gpu["passes"])
+                cpu = run_cpu_baseline(args, cf)
             except Exception as e:  # reported, never silently replaced
                 log(f"cpu baseline failed: {e!r}")
                 cpu = {"value": None, "error": repr(e)}
 
-    circuit_s = gpu["circuit_ms"] / 1e3
+    plain = results.get("plain-fp32-vector")
     out = {
         "metric": METRIC,
-        "value": circuit_s,
+        "value": b["circuit_s"],
         "unit": "s/circuit",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": gpu["circuit_ms"],
+        "ms_per_step": b["circuit_s"] * 1e3,
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
@@ -206,20 +290,19 @@ def main():
         "config": {
             "workload": (f"{args.qubits}q random grid circuit {rows}x{cols}, "
                          f"{args.cycles} cycles, seed {args.seed}, complex64, "
-                         f"MultiQubitGateFuser max_fused_size={args.max_fused}, "
-                         "plain FP32 vector kernels"),
+                         f"path {best}"),
             "num_qubits": args.qubits, "cycles": args.cycles,
-            "max_fused_size": args.max_fused, "method": "plain-fp32",
-            "num_gates": gpu["num_gates"], "fused_passes": gpu["passes"],
+            "method": best, "max_fused_size": b["max_fused_size"],
+            "num_gates": b["num_gates"], "fused_passes": b["passes"],
         },
-        "gate_passes_per_s": gpu["passes"] / circuit_s,
-        "hbm_gbps_avg_pass": 16.0 * (1 << gpu["num_qubits"]) /
-                             (gpu["all_pass_avg_ms"] * 1e-3) / 1e9,
-        "pass_ms_sum": gpu["pass_ms_sum"],
-        "final_norm": gpu["norm"],
-        "kernels": gpu["families"],
-        "per_k": gpu["per_k"],
-        "roofline": gpu["roofline"],
+        "gate_passes_per_s": b["gate_passes_per_s"],
+        "hbm_gbps_avg_pass": b["hbm_gbps_avg_pass"],
+        "speedup_vs_plain_fp32_k4": (plain["circuit_s"] / b["circuit_s"]
+                                     if plain else None),
+        "fidelity": fid.get(best),
+        "paths": {p: {k: v for k, v in r.items() if k != "roofline"}
+                  for p, r in results.items()},
+        "roofline": b["roofline"],
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

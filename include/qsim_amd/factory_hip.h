@@ -17,6 +17,7 @@ template <typename FP = float>
 struct FactoryHIP {
   using Simulator = SimulatorHIP<FP>;
   using StateSpace = typename Simulator::StateSpace;
+  using fp_type = FP;
 
   FactoryHIP() {}
   explicit FactoryHIP(const typename StateSpace::Parameter&) {}

@@ -88,6 +88,12 @@ int qsim_hip_copy_d2d(void* dst, const void* src, uint64_t bytes);
 int qsim_hip_copy_d2h(void* dst, const void* src, uint64_t bytes);
 int qsim_hip_copy_h2d(void* dst, const void* src, uint64_t bytes);
 
+/* Copies an n-qubit state between element types (complex64 <-> complex128,
+ * or the same type), rounding to nearest.  No reference counterpart: it lets
+ * a complex64 run be compared with a complex128 run on the device. */
+int qsim_hip_copy_convert(void* dst, int dst_prec, const void* src,
+                          int src_prec, unsigned num_qubits);
+
 /* ---- StateSpace (lib/statespace_cuda.h) -------------------------------- */
 
 /* All amplitudes 0 (lib/statespace_cuda.h:109-112). */

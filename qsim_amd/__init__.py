@@ -54,6 +54,7 @@ SIGNATURES = [
     ("qsim_hip_copy_d2d", _i, [_vp, _vp, _u64]),
     ("qsim_hip_copy_d2h", _i, [_vp, _vp, _u64]),
     ("qsim_hip_copy_h2d", _i, [_vp, _vp, _u64]),
+    ("qsim_hip_copy_convert", _i, [_vp, _i, _vp, _i, _u]),
     ("qsim_hip_set_all_zeros", _i, [_i, _vp, _u]),
     ("qsim_hip_set_state_zero", _i, [_i, _vp, _u]),
     ("qsim_hip_set_state_uniform", _i, [_i, _vp, _u]),
@@ -261,6 +262,10 @@ class StateSpace:
     def copy(self, src: State, dest: State) -> bool:
         if src.num_qubits != dest.num_qubits:
             return False
+        if src.prec != dest.prec:
+            _check(lib().qsim_hip_copy_convert(dest.ptr, dest.prec, src.ptr,
+                                               src.prec, src.num_qubits))
+            return True
         _check(lib().qsim_hip_copy_d2d(dest.ptr, src.ptr, src.nbytes))
         return True
 

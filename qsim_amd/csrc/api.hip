@@ -279,6 +279,13 @@ int qsim_hip_copy_h2d(void* dst, const void* src, uint64_t bytes) {
   return Copy(dst, src, bytes, hipMemcpyHostToDevice);
 }
 
+int qsim_hip_copy_convert(void* dst, int dst_prec, const void* src,
+                          int src_prec, unsigned n) {
+  QHIP_TRY(CheckState(dst_prec, dst, n));
+  QHIP_TRY(CheckState(src_prec, src, n));
+  return Convert(dst, dst_prec, src, src_prec, n);
+}
+
 #define QHIP_DISPATCH(prec, F32CALL, F64CALL) \
   ((prec) == QSIM_HIP_F32 ? (F32CALL) : (F64CALL))
 

@@ -147,6 +147,8 @@ template <typename FP>
 int Sample(const CT<FP>* s, unsigned n, const double* draws, uint64_t num,
            uint64_t* out);
 
+int Convert(void* dst, int dprec, const void* src, int sprec, unsigned n);
+
 // MFMA path for uncontrolled k>=5 gates on complex64 states.  `m64` is the
 // matrix in FP64 (rounded products are formed from it); returns
 // QSIM_HIP_ERR_UNSUPPORTED when the method cannot handle the gate.

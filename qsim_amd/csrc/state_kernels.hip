@@ -478,6 +478,37 @@ int Sample(const C_t<FP>* s, unsigned n, const double* draws, uint64_t num,
   template int Sample<FP>(const C_t<FP>*, unsigned, const double*, uint64_t,   \
                           uint64_t*);
 
+template <typename D, typename S>
+__global__ void __launch_bounds__(kBS)
+ConvertKernel(D* __restrict__ dst, const S* __restrict__ src, uint64_t size) {
+  const uint64_t stride = uint64_t(gridDim.x) * kBS;
+  for (uint64_t i = uint64_t(blockIdx.x) * kBS + threadIdx.x; i < size;
+       i += stride) {
+    S v = src[i];
+    dst[i] = D{decltype(dst->x)(v.x), decltype(dst->x)(v.y)};
+  }
+}
+
+int Convert(void* dst, int dprec, const void* src, int sprec, unsigned n) {
+  const uint64_t size = uint64_t{1} << n;
+  const dim3 grid(GridFor(size));
+  if (dprec == QSIM_HIP_F64 && sprec == QSIM_HIP_F32) {
+    hipLaunchKernelGGL((ConvertKernel<double2, float2>), grid, dim3(kBS), 0,
+                       Stream(), (double2*)dst, (const float2*)src, size);
+  } else if (dprec == QSIM_HIP_F32 && sprec == QSIM_HIP_F64) {
+    hipLaunchKernelGGL((ConvertKernel<float2, double2>), grid, dim3(kBS), 0,
+                       Stream(), (float2*)dst, (const double2*)src, size);
+  } else if (dprec == QSIM_HIP_F32) {
+    hipLaunchKernelGGL((ConvertKernel<float2, float2>), grid, dim3(kBS), 0,
+                       Stream(), (float2*)dst, (const float2*)src, size);
+  } else {
+    hipLaunchKernelGGL((ConvertKernel<double2, double2>), grid, dim3(kBS), 0,
+                       Stream(), (double2*)dst, (const double2*)src, size);
+  }
+  QHIP_LAUNCHED("ConvertKernel");
+  return QSIM_HIP_OK;
+}
+
 QHIP_INSTANTIATE(float)
 QHIP_INSTANTIATE(double)
 
