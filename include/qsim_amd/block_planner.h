@@ -1,0 +1,238 @@
+// block_planner.h -- partitions a qsim circuit into cache-blocked passes.
+//
+// A plain fused pass (MultiQubitGateFuser, lib/fuser_mqubit.h:185-526) costs
+// one stream of the state through HBM per fused gate of <= 6 qubits.  A
+// blocked pass (qsim_hip_apply_gate_block) costs one stream per tile qubit
+// set of up to 13 qubits: the Lc lowest qubits plus chosen higher ones.  The
+// planner walks the circuit's dependency frontier and, pass by pass, grows
+// the tile set greedily -- adding the single qubit, or the missing qubits of
+// a frontier gate, that unlocks the most gates per added qubit -- until the
+// set is full; every gate whose qubits lie in the set and whose
+// predecessors are applied joins the pass.  The gates of a pass are then
+// fused into <= 4-qubit gates by qsim's own MultiQubitGateFuser, with
+// matrices multiplied in the operations' FP type (FP64 for the blocked path).
+//
+// Operations that are not uncontrolled gates on <= 4 qubits (measurements,
+// controlled gates, channels, larger gates) end the current segment and are
+// handed back as single operations, in order.
+#ifndef QSIM_AMD_BLOCK_PLANNER_H_
+#define QSIM_AMD_BLOCK_PLANNER_H_
+
+#include <algorithm>
+#include <cstdint>
+#include <set>
+#include <variant>
+#include <vector>
+
+#include "fuser_mqubit.h"
+#include "gate.h"
+#include "operation.h"
+
+namespace qsim {
+namespace hip {
+
+struct BlockPass {
+  std::vector<unsigned> tile;  // tile qubits, ascending
+  std::vector<unsigned> ks;    // per fused gate: qubit count (<= 4)
+  std::vector<unsigned> qs;    // per fused gate: 4 slots, ascending qubits
+  std::vector<double> m;       // fused matrices, concatenated (2 * 4^k each)
+  unsigned num_gates = 0;      // circuit gates in the pass
+};
+
+// One step of a blocked plan: a blocked pass, or operation `op` of the
+// circuit applied on its own.
+struct BlockStep {
+  bool blocked;
+  size_t index;  // into BlockPlan::blocks, or into the circuit's ops
+};
+
+struct BlockPlan {
+  std::vector<BlockPass> blocks;
+  std::vector<BlockStep> steps;
+};
+
+struct BlockPlannerParameter {
+  unsigned tile_bits = 13;  // tile qubits per pass (9..13)
+  unsigned low_bits = 5;    // lowest qubits always in the tile (>= 3)
+  unsigned max_ops = 32;    // fused gates per pass
+};
+
+namespace detail {
+
+template <typename FP>
+const Gate<FP>* BlockableGate(const Operation<FP>& op) {
+  const Gate<FP>* g = OpGetAlternative<Gate<FP>>(op);
+  if (g == nullptr || g->qubits.empty() || g->qubits.size() > 4) return nullptr;
+  return g;
+}
+
+// Gate dependency frontier over a segment of blockable gates.
+class Frontier {
+ public:
+  Frontier(unsigned n, const std::vector<std::vector<unsigned>>& gq)
+      : gq_(gq), per_q_(n), ptr_(n, 0) {
+    for (size_t i = 0; i < gq.size(); ++i)
+      for (unsigned q : gq[i]) per_q_[q].push_back(i);
+  }
+
+  bool Done() const {
+    for (size_t q = 0; q < per_q_.size(); ++q)
+      if (ptr_[q] < per_q_[q].size()) return false;
+    return true;
+  }
+
+  // Gates that become applicable with qubit set `in` (a bit mask over
+  // qubits), in closure order; advances `ptr` accordingly.
+  size_t Closure(const std::vector<char>& in, std::vector<size_t>& ptr,
+                 std::vector<size_t>* taken) const {
+    size_t cnt = 0;
+    bool changed = true;
+    while (changed) {
+      changed = false;
+      for (size_t q = 0; q < per_q_.size(); ++q) {
+        if (!in[q]) continue;
+        while (ptr[q] < per_q_[q].size()) {
+          const size_t i = per_q_[q][ptr[q]];
+          bool ok = true;
+          for (unsigned x : gq_[i]) {
+            ok = ok && in[x] && ptr[x] < per_q_[x].size() &&
+                 per_q_[x][ptr[x]] == i;
+          }
+          if (!ok) break;
+          for (unsigned x : gq_[i]) ++ptr[x];
+          if (taken) taken->push_back(i);
+          ++cnt;
+          changed = true;
+        }
+      }
+    }
+    return cnt;
+  }
+
+  // Chooses a tile set (low qubits + greedy additions) and takes the gates it
+  // unlocks.  Returns them in circuit order (ascending index).
+  std::vector<size_t> NextPass(unsigned tile_bits, unsigned low_bits,
+                               std::vector<unsigned>& tile) {
+    const unsigned n = per_q_.size();
+    std::vector<char> in(n, 0);
+    unsigned size = 0;
+    for (unsigned q = 0; q < low_bits && q < n; ++q) {
+      in[q] = 1;
+      ++size;
+    }
+    while (size < tile_bits) {
+      std::vector<size_t> p0 = ptr_;
+      const size_t base = Closure(in, p0, nullptr);
+      // Candidates: single qubits, and the missing qubits of frontier gates.
+      std::vector<std::vector<unsigned>> cands;
+      for (unsigned q = 0; q < n; ++q)
+        if (!in[q]) cands.push_back({q});
+      std::set<std::vector<unsigned>> seen;
+      for (unsigned q = 0; q < n; ++q) {
+        if (p0[q] >= per_q_[q].size()) continue;
+        std::vector<unsigned> add;
+        for (unsigned x : gq_[per_q_[q][p0[q]]])
+          if (!in[x]) add.push_back(x);
+        if (add.size() > 1 && seen.insert(add).second) cands.push_back(add);
+      }
+      double best = -1;
+      const std::vector<unsigned>* pick = nullptr;
+      for (const auto& c : cands) {
+        if (size + c.size() > tile_bits) continue;
+        for (unsigned x : c) in[x] = 1;
+        std::vector<size_t> p1 = ptr_;
+        const double gain =
+            double(Closure(in, p1, nullptr) - base) / double(c.size());
+        for (unsigned x : c) in[x] = 0;
+        if (gain > best) {
+          best = gain;
+          pick = &c;
+        }
+      }
+      if (pick == nullptr) break;
+      for (unsigned x : *pick) in[x] = 1;
+      size += pick->size();
+    }
+    std::vector<size_t> taken;
+    Closure(in, ptr_, &taken);
+    std::sort(taken.begin(), taken.end());
+    tile.clear();
+    for (unsigned q = 0; q < n; ++q)
+      if (in[q]) tile.push_back(q);
+    return taken;
+  }
+
+ private:
+  const std::vector<std::vector<unsigned>>& gq_;
+  std::vector<std::vector<size_t>> per_q_;
+  std::vector<size_t> ptr_;
+};
+
+}  // namespace detail
+
+// Plans blocked passes for ops of a circuit on `num_qubits` qubits.  Returns
+// false if the circuit is too small for blocked passes (num_qubits < 9).
+template <typename IO, typename FP>
+bool PlanBlocks(const BlockPlannerParameter& param, unsigned num_qubits,
+                const std::vector<Operation<FP>>& ops, BlockPlan& plan) {
+  plan = BlockPlan{};
+  const unsigned tb = std::min(param.tile_bits, num_qubits);
+  if (tb < 9 || tb > 13) return false;
+  const unsigned lc = std::min(std::max(param.low_bits, 3u), tb);
+
+  size_t i = 0;
+  while (i < ops.size()) {
+    // A segment of blockable gates [i, j).
+    size_t j = i;
+    while (j < ops.size() && detail::BlockableGate(ops[j]) != nullptr) ++j;
+    if (j > i) {
+      std::vector<std::vector<unsigned>> gq;
+      for (size_t t = i; t < j; ++t)
+        gq.push_back(detail::BlockableGate(ops[t])->qubits);
+      detail::Frontier fr(num_qubits, gq);
+      while (!fr.Done()) {
+        std::vector<unsigned> tile;
+        std::vector<size_t> taken = fr.NextPass(tb, lc, tile);
+        if (taken.empty()) return false;  // cannot happen for k <= 4 gates
+        std::vector<const Operation<FP>*> ptrs;
+        for (size_t t : taken) ptrs.push_back(&ops[i + t]);
+        typename MultiQubitGateFuser<IO>::Parameter fp;
+        fp.max_fused_size = 4;
+        fp.verbosity = 0;
+        const std::vector<unsigned> no_splits;
+        auto fused =
+            MultiQubitGateFuser<IO>::template FuseGates<const Operation<FP>*>(
+                fp, num_qubits, ptrs.cbegin(), ptrs.cend(), no_splits, true);
+        BlockPass bp;
+        bp.tile = tile;
+        bp.num_gates = taken.size();
+        auto flush = [&]() {
+          plan.steps.push_back({true, plan.blocks.size()});
+          plan.blocks.push_back(bp);
+          bp.ks.clear();
+          bp.qs.clear();
+          bp.m.clear();
+          bp.num_gates = 0;
+        };
+        for (const auto& f : fused) {
+          const auto* fg = std::get_if<FusedGate<FP>>(&f);
+          if (fg == nullptr) return false;  // only gates were passed in
+          if (bp.ks.size() == param.max_ops) flush();
+          bp.ks.push_back(fg->qubits.size());
+          for (unsigned b = 0; b < 4; ++b)
+            bp.qs.push_back(b < fg->qubits.size() ? fg->qubits[b] : 0);
+          for (auto x : fg->matrix) bp.m.push_back(double(x));
+        }
+        if (!bp.ks.empty()) flush();
+      }
+    }
+    if (j < ops.size()) plan.steps.push_back({false, j});
+    i = j + 1;
+  }
+  return true;
+}
+
+}  // namespace hip
+}  // namespace qsim
+
+#endif  // QSIM_AMD_BLOCK_PLANNER_H_

@@ -40,8 +40,9 @@ enum {
    * sums); everything else: the FP32 / FP64 vector kernels */
   QSIM_HIP_METHOD_AUTO = 0,
   QSIM_HIP_METHOD_VECTOR = 1,       /* always the vector kernels */
-  QSIM_HIP_METHOD_MFMA_BF16X3 = 2,  /* as AUTO, but k >= 5 on split-BF16 x3
-                                       MFMA with FP32 accumulation */
+  QSIM_HIP_METHOD_MFMA_BF16X3 = 2,  /* as AUTO, but k >= 5 on split-BF16
+                                       MFMA (matrix in 3 bf16 parts, state in
+                                       2, 4 products) with FP32 accumulation */
   QSIM_HIP_METHOD_MFMA_FP16X2 = 3   /* reserved: scaled two-term FP16 MFMA
                                        (selecting it is rejected) */
 };
@@ -195,6 +196,24 @@ int qsim_hip_apply_controlled_gate(int prec, void* state, unsigned num_qubits,
                                    const unsigned* cqs, unsigned nc,
                                    uint64_t cvals, const void* matrix,
                                    int matrix_prec);
+
+/* A cache-blocked pass: applies num_gates (<= 32) gates in order with one
+ * stream of the state through HBM.  The state is cut into tiles over the
+ * tile qubits (ascending, 9..13 of them, starting with the run 0, 1, 2, ...
+ * of at least 3 qubits, at most 10 beyond it); gate i acts on
+ * gate_num_qubits[i] (1..4) ascending qubits gate_qubits[4 i ...], all tile
+ * qubits, with the next 2 * 4^k matrix elements of `matrices` (row-major
+ * interleaved, bit j <-> qubit j).  Arithmetic: split-BF16 MFMA products
+ * (matrix in 3 bf16 parts, state in 2) with FP32 accumulation.  Complex64 states only (else
+ * QSIM_HIP_ERR_UNSUPPORTED).  Equivalent to one SimulatorCUDA::ApplyGate
+ * (lib/simulator_cuda.h:70-125) per gate, in the apply loop of
+ * QSimRunner::Run (lib/run_qsim.h:264-280). */
+int qsim_hip_apply_gate_block(int prec, void* state, unsigned num_qubits,
+                              const unsigned* tile_qubits,
+                              unsigned num_tile_qubits, unsigned num_gates,
+                              const unsigned* gate_num_qubits,
+                              const unsigned* gate_qubits,
+                              const void* matrices, int matrix_prec);
 
 /* <psi|U|psi> for k <= 6 with FP64 accumulation.  Replaces SimulatorCUDA::
  * ExpectationValue (lib/simulator_cuda.h:216-260). */

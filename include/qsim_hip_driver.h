@@ -35,11 +35,39 @@ unsigned qsim_driver_num_gates(void* circuit);
 int qsim_driver_fuse(void* circuit, unsigned max_fused_size, int fuser,
                      int matrix_prec);
 
+/* Partitions the circuit into cache-blocked passes of `tile_bits` (9..13)
+ * tile qubits that always include the `low_bits` (3..tile_bits) lowest
+ * qubits (qsim_amd/block_planner.h); the gates of a pass are fused to
+ * <= 4 qubits by MultiQubitGateFuser with FP64 matrices.  Operations that
+ * are not uncontrolled gates on <= 4 qubits stay single passes.  No
+ * reference counterpart: it replaces the partition of FuseGates
+ * (lib/fuser_mqubit.h:185-526) and the per-gate loop of QSimRunner::Run
+ * (lib/run_qsim.h:264-280).  QSIM_HIP_ERR_UNSUPPORTED below 9 qubits. */
+int qsim_driver_fuse_blocked(void* circuit, unsigned tile_bits,
+                             unsigned low_bits);
+
 unsigned qsim_driver_num_passes(void* circuit);
 
-/* Pass i: its qubit count, qubits (ascending, up to 6) and gate count. */
+/* Kinds of passes (qsim_driver_pass_kind). */
+enum {
+  QSIM_DRIVER_PASS_GATE = 0,         /* a fused gate */
+  QSIM_DRIVER_PASS_MEASUREMENT = 1,  /* a (fused) measurement */
+  QSIM_DRIVER_PASS_OPERATION = 2,    /* an unfused operation (controlled gate) */
+  QSIM_DRIVER_PASS_BLOCKED = 3       /* a cache-blocked pass */
+};
+int qsim_driver_pass_kind(void* circuit, unsigned i);
+
+/* Pass i: its qubit count, qubits (ascending, up to 64) and gate count.  For
+ * a blocked pass: the tile qubits and the circuit gates it applies. */
 int qsim_driver_pass_info(void* circuit, unsigned i, unsigned* k,
                           unsigned* qubits, unsigned* num_gates);
+
+/* Blocked pass i: its fused gates (<= 32): qubit counts, 4 qubit slots
+ * each and, if op_matrices is not NULL, their FP64 matrices concatenated
+ * (2 * 4^k doubles each; at most 32 * 512). */
+int qsim_driver_block_info(void* circuit, unsigned i, unsigned* num_ops,
+                           unsigned* op_k, unsigned* op_qubits,
+                           double* op_matrices);
 
 /* Applies passes [first, first + count) to a state of the circuit's size
  * (the apply loop of QSimRunner::Run, lib/run_qsim.h:264-280); measurement

@@ -77,6 +77,8 @@ SIGNATURES = [
     ("qsim_hip_apply_gate", _i, [_i, _vp, _u, _u32p, _u, _vp, _i]),
     ("qsim_hip_apply_controlled_gate", _i, [_i, _vp, _u, _u32p, _u, _u32p, _u,
                                             _u64, _vp, _i]),
+    ("qsim_hip_apply_gate_block", _i, [_i, _vp, _u, _u32p, _u, _u, _u32p,
+                                       _u32p, _vp, _i]),
     ("qsim_hip_expectation_value", _i, [_i, _vp, _u, _u32p, _u, _vp, _i,
                                         _f64p, _f64p]),
     ("qsim_hip_set_gate_method", _i, [_i]),
@@ -400,6 +402,24 @@ class Simulator:
         _check(lib().qsim_hip_apply_controlled_gate(
             s.prec, s.ptr, s.num_qubits, qp, len(qarr), cp, len(carr), cvals,
             m.ctypes.data, mprec))
+
+    def apply_gate_block(self, tile_qubits, gates, s: State):
+        """Applies `gates` [(qs, matrix), ...] (k <= 4, qubits inside
+        `tile_qubits`) in one cache-blocked pass; FP64 or FP32 matrices."""
+        tarr, tp = _u32(sorted(tile_qubits))
+        ks, qs, ms = [], [], []
+        for q, m in gates:
+            q = list(q)
+            ks.append(len(q))
+            qs.extend(q + [0] * (4 - len(q)))
+            ms.append(np.asarray(m, dtype=np.float64).reshape(-1))
+        karr, kp = _u32(ks)
+        qarr, qp = _u32(qs)
+        mat = (np.ascontiguousarray(np.concatenate(ms)) if ms
+               else np.zeros(2, np.float64))
+        _check(lib().qsim_hip_apply_gate_block(
+            s.prec, s.ptr, s.num_qubits, tp, len(tarr), len(ks), kp, qp,
+            mat.ctypes.data, F64))
 
     def expectation_value(self, qs, matrix, s: State) -> complex:
         qarr, qp = _u32(qs)

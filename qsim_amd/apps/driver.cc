@@ -2,9 +2,10 @@
 // backend (C ABI in include/qsim_hip_driver.h).
 //
 // Parsing is qsim's CircuitQsimParser, the partition is qsim's
-// MultiQubitGateFuser (or the cost-model fuser), and the apply loop is the one
-// of QSimRunner::Run (lib/run_qsim.h:264-280) with gate_appl.h's dispatch, on
-// SimulatorHIP / StateSpaceHIP.  Compiled against qsim's lib/ headers.
+// MultiQubitGateFuser or the blocked-pass planner (qsim_amd/block_planner.h),
+// and the apply loop is the one of QSimRunner::Run (lib/run_qsim.h:264-280)
+// with gate_appl.h's dispatch, on SimulatorHIP / StateSpaceHIP.  Compiled
+// against qsim's lib/ headers.
 
 #include <cstdint>
 #include <cstring>
@@ -19,6 +20,7 @@
 #include "fuser_mqubit.h"
 #include "gate_appl.h"
 #include "io.h"
+#include "qsim_amd/block_planner.h"
 #include "qsim_amd/simulator_hip.h"
 #include "qsim_hip.h"
 #include "qsim_hip_driver.h"
@@ -31,13 +33,28 @@ template <typename FP>
 using FusedOps =
     std::vector<std::variant<FusedGate<FP>, Measurement, const Operation<FP>*>>;
 
+// A pass of the current partition: entry `index` of f32 / f64 (a fused
+// gate, a measurement or an unfused operation), or blocked pass `index`.
+struct Step {
+  bool blocked;
+  size_t index;
+};
+
 struct DriverCircuit {
   Circuit<Operation<float>> c32;
   Circuit<Operation<double>> c64;
   int matrix_prec = QSIM_HIP_F32;
   FusedOps<float> f32;
   FusedOps<double> f64;
+  std::vector<hip::BlockPass> blocks;
+  std::vector<Step> steps;
 };
+
+void PlainSteps(DriverCircuit& d) {
+  d.blocks.clear();
+  d.steps.clear();
+  for (size_t i = 0; i < d.f32.size(); ++i) d.steps.push_back({false, i});
+}
 
 template <typename FP>
 bool Parse(const std::string& text, unsigned maxtime,
@@ -81,6 +98,12 @@ void PassInfo(const FusedOp& op, unsigned* k, unsigned* qubits,
   if (num_gates) *num_gates = ng;
 }
 
+int RunBlock(const hip::BlockPass& b, float* p, unsigned n) {
+  return qsim_hip_apply_gate_block(QSIM_HIP_F32, p, n, b.tile.data(),
+                                   b.tile.size(), b.ks.size(), b.ks.data(),
+                                   b.qs.data(), b.m.data(), QSIM_HIP_F64);
+}
+
 // Complex64 state, FP32 matrices: exactly QSimRunner's loop.
 int RunF32(DriverCircuit& d, float* p, unsigned first, unsigned count,
            unsigned seed) {
@@ -89,7 +112,13 @@ int RunF32(DriverCircuit& d, float* p, unsigned first, unsigned count,
   auto state = ss.Create(p, d.c32.num_qubits);
   std::mt19937 rgen(seed);
   for (unsigned i = first; i < first + count; ++i) {
-    if (!ApplyGate(ss, sim, d.f32[i], rgen, state)) return QSIM_HIP_ERR_ARG;
+    const Step& st = d.steps[i];
+    if (st.blocked) {
+      int rc = RunBlock(d.blocks[st.index], p, d.c32.num_qubits);
+      if (rc != QSIM_HIP_OK) return rc;
+    } else if (!ApplyGate(ss, sim, d.f32[st.index], rgen, state)) {
+      return QSIM_HIP_ERR_ARG;
+    }
   }
   return QSIM_HIP_OK;
 }
@@ -103,7 +132,13 @@ int RunF32M64(DriverCircuit& d, float* p, unsigned first, unsigned count,
   std::mt19937 rgen(seed);
   const unsigned n = d.c64.num_qubits;
   for (unsigned i = first; i < first + count; ++i) {
-    const auto& op = d.f64[i];
+    const Step& st = d.steps[i];
+    if (st.blocked) {
+      int rc = RunBlock(d.blocks[st.index], p, n);
+      if (rc != QSIM_HIP_OK) return rc;
+      continue;
+    }
+    const auto& op = d.f64[st.index];
     if (const auto* fg = std::get_if<FusedGate<double>>(&op)) {
       sim.ApplyGate(fg->qubits, fg->matrix.data(), state);
     } else if (const auto* m = std::get_if<Measurement>(&op)) {
@@ -125,15 +160,64 @@ int RunF32M64(DriverCircuit& d, float* p, unsigned first, unsigned count,
   return QSIM_HIP_OK;
 }
 
-// Complex128 state.
+// Complex128 state.  Blocked passes run gate by gate (FP64 arithmetic).
 int RunF64(DriverCircuit& d, double* p, unsigned first, unsigned count,
            unsigned seed) {
   StateSpaceHIP<double> ss;
   SimulatorHIP<double> sim;
   auto state = ss.Create(p, d.c64.num_qubits);
   std::mt19937 rgen(seed);
+  const unsigned n = d.c64.num_qubits;
   for (unsigned i = first; i < first + count; ++i) {
-    if (!ApplyGate(ss, sim, d.f64[i], rgen, state)) return QSIM_HIP_ERR_ARG;
+    const Step& st = d.steps[i];
+    if (st.blocked) {
+      const hip::BlockPass& b = d.blocks[st.index];
+      size_t off = 0;
+      for (size_t g = 0; g < b.ks.size(); ++g) {
+        int rc = qsim_hip_apply_gate(QSIM_HIP_F64, p, n, &b.qs[4 * g],
+                                     b.ks[g], &b.m[off], QSIM_HIP_F64);
+        if (rc != QSIM_HIP_OK) return rc;
+        off += size_t{2} << (2 * b.ks[g]);
+      }
+    } else if (!ApplyGate(ss, sim, d.f64[st.index], rgen, state)) {
+      return QSIM_HIP_ERR_ARG;
+    }
+  }
+  return QSIM_HIP_OK;
+}
+
+template <typename FP>
+int PlanBlocked(const Circuit<Operation<FP>>& c, unsigned tile_bits,
+                unsigned low_bits, std::vector<hip::BlockPass>& blocks,
+                std::vector<Step>& steps, FusedOps<FP>& single) {
+  hip::BlockPlannerParameter bp;
+  bp.tile_bits = tile_bits;
+  bp.low_bits = low_bits;
+  hip::BlockPlan plan;
+  if (!hip::PlanBlocks<IO>(bp, c.num_qubits, c.ops, plan)) {
+    return QSIM_HIP_ERR_UNSUPPORTED;
+  }
+  blocks = std::move(plan.blocks);
+  steps.clear();
+  single.clear();
+  // Operations outside blocked passes go through qsim's fuser one by one,
+  // which turns them into the entries its apply loop expects.
+  MultiQubitGateFuser<IO>::Parameter p;
+  p.max_fused_size = 2;
+  p.verbosity = 0;
+  for (const auto& s : plan.steps) {
+    if (s.blocked) {
+      steps.push_back({true, s.index});
+      continue;
+    }
+    const std::vector<unsigned> no_splits;
+    auto f = MultiQubitGateFuser<IO>::template FuseGates<Operation<FP>>(
+        p, c.num_qubits, c.ops.cbegin() + s.index,
+        c.ops.cbegin() + s.index + 1, no_splits, true);
+    for (auto& e : f) {
+      steps.push_back({false, single.size()});
+      single.push_back(std::move(e));
+    }
   }
   return QSIM_HIP_OK;
 }
@@ -165,27 +249,99 @@ int qsim_driver_fuse(void* c, unsigned max_fused_size, int fuser,
   auto* d = static_cast<DriverCircuit*>(c);
   d->matrix_prec = matrix_prec;
   int rc = Fuse(d->c32, max_fused_size, fuser, d->f32);
-  if (rc != QSIM_HIP_OK) return rc;
-  return Fuse(d->c64, max_fused_size, fuser, d->f64);
+  if (rc == QSIM_HIP_OK) rc = Fuse(d->c64, max_fused_size, fuser, d->f64);
+  if (rc != QSIM_HIP_OK) {
+    d->f32.clear();
+    d->f64.clear();
+  }
+  PlainSteps(*d);
+  return rc;
+}
+
+int qsim_driver_fuse_blocked(void* c, unsigned tile_bits, unsigned low_bits) {
+  auto* d = static_cast<DriverCircuit*>(c);
+  if (tile_bits < 9 || tile_bits > 13 || low_bits < 3 ||
+      low_bits > tile_bits) {
+    return QSIM_HIP_ERR_ARG;
+  }
+  d->matrix_prec = QSIM_HIP_F64;
+  std::vector<hip::BlockPass> b32;
+  std::vector<Step> s32;
+  int rc = PlanBlocked(d->c64, tile_bits, low_bits, d->blocks, d->steps,
+                       d->f64);
+  if (rc == QSIM_HIP_OK) {
+    rc = PlanBlocked(d->c32, tile_bits, low_bits, b32, s32, d->f32);
+  }
+  if (rc == QSIM_HIP_OK && s32.size() != d->steps.size()) {
+    rc = QSIM_HIP_ERR_RUNTIME;
+  }
+  if (rc != QSIM_HIP_OK) {
+    d->f32.clear();
+    d->f64.clear();
+    PlainSteps(*d);
+  }
+  return rc;
 }
 
 unsigned qsim_driver_num_passes(void* c) {
-  return static_cast<DriverCircuit*>(c)->f32.size();
+  return static_cast<DriverCircuit*>(c)->steps.size();
+}
+
+int qsim_driver_pass_kind(void* c, unsigned i) {
+  auto* d = static_cast<DriverCircuit*>(c);
+  if (i >= d->steps.size()) return QSIM_HIP_ERR_ARG;
+  const Step& st = d->steps[i];
+  if (st.blocked) return QSIM_DRIVER_PASS_BLOCKED;
+  const auto& op = d->f32[st.index];
+  if (std::get_if<FusedGate<float>>(&op)) return QSIM_DRIVER_PASS_GATE;
+  if (std::get_if<Measurement>(&op)) return QSIM_DRIVER_PASS_MEASUREMENT;
+  // Unfusible operations (measurements included) come back as pointers to
+  // the circuit's own operations.
+  const Operation<float>* o = std::get<const Operation<float>*>(op);
+  if (OpGetAlternative<Measurement>(*o)) return QSIM_DRIVER_PASS_MEASUREMENT;
+  return QSIM_DRIVER_PASS_OPERATION;
 }
 
 int qsim_driver_pass_info(void* c, unsigned i, unsigned* k, unsigned* qubits,
                           unsigned* num_gates) {
   auto* d = static_cast<DriverCircuit*>(c);
-  if (i >= d->f32.size()) return QSIM_HIP_ERR_ARG;
-  PassInfo<float>(d->f32[i], k, qubits, num_gates);
+  if (i >= d->steps.size()) return QSIM_HIP_ERR_ARG;
+  const Step& st = d->steps[i];
+  if (st.blocked) {
+    const hip::BlockPass& b = d->blocks[st.index];
+    *k = b.tile.size();
+    if (qubits) {
+      for (size_t j = 0; j < b.tile.size(); ++j) qubits[j] = b.tile[j];
+    }
+    if (num_gates) *num_gates = b.num_gates;
+    return QSIM_HIP_OK;
+  }
+  PassInfo<float>(d->f32[st.index], k, qubits, num_gates);
+  return QSIM_HIP_OK;
+}
+
+int qsim_driver_block_info(void* c, unsigned i, unsigned* num_ops,
+                           unsigned* op_k, unsigned* op_qubits,
+                           double* op_matrices) {
+  auto* d = static_cast<DriverCircuit*>(c);
+  if (i >= d->steps.size() || !d->steps[i].blocked) return QSIM_HIP_ERR_ARG;
+  const hip::BlockPass& b = d->blocks[d->steps[i].index];
+  *num_ops = b.ks.size();
+  for (size_t g = 0; g < b.ks.size(); ++g) {
+    if (op_k) op_k[g] = b.ks[g];
+    if (op_qubits) {
+      for (int j = 0; j < 4; ++j) op_qubits[4 * g + j] = b.qs[4 * g + j];
+    }
+  }
+  if (op_matrices) std::memcpy(op_matrices, b.m.data(), b.m.size() * 8);
   return QSIM_HIP_OK;
 }
 
 int qsim_driver_run_passes(void* c, void* state, int prec, unsigned first,
                            unsigned count, unsigned seed) {
   auto* d = static_cast<DriverCircuit*>(c);
-  if (state == nullptr || first > d->f32.size() ||
-      count > d->f32.size() - first || d->f32.size() != d->f64.size()) {
+  if (state == nullptr || first > d->steps.size() ||
+      count > d->steps.size() - first || d->f32.size() != d->f64.size()) {
     return QSIM_HIP_ERR_ARG;
   }
   if (prec == QSIM_HIP_F64) {

@@ -27,7 +27,9 @@ struct Device {
   size_t scratch_size = 0;
   // Matrix ring: device slots and pinned host staging, one event per slot.
   static constexpr int kSlots = 64;
-  static constexpr size_t kSlotBytes = 64 * 1024;  // k = 6 FP64 matrix
+  // A k = 6 FP64 matrix or BF16 fragments, or the fragments of a full
+  // cache-blocked program (32 x 6 KiB).
+  static constexpr size_t kSlotBytes = 192 * 1024;
   char* mat_dev = nullptr;
   char* mat_host = nullptr;
   hipEvent_t mat_ev[kSlots] = {};
@@ -515,6 +517,52 @@ int qsim_hip_apply_gate(int prec, void* s, unsigned n, const unsigned* qs,
                         unsigned k, const void* matrix, int matrix_prec) {
   return qsim_hip_apply_controlled_gate(prec, s, n, qs, k, nullptr, 0, 0,
                                         matrix, matrix_prec);
+}
+
+int qsim_hip_apply_gate_block(int prec, void* s, unsigned n,
+                              const unsigned* tile_qubits,
+                              unsigned num_tile_qubits, unsigned num_gates,
+                              const unsigned* gate_num_qubits,
+                              const unsigned* gate_qubits,
+                              const void* matrices, int matrix_prec) {
+  QHIP_TRY(CheckState(prec, s, n));
+  if (tile_qubits == nullptr ||
+      (num_gates > 0 && (gate_num_qubits == nullptr ||
+                         gate_qubits == nullptr || matrices == nullptr))) {
+    SetError("null blocked-pass operand");
+    return QSIM_HIP_ERR_ARG;
+  }
+  if (prec != QSIM_HIP_F32) {
+    SetError("blocked passes run on complex64 states only");
+    return QSIM_HIP_ERR_UNSUPPORTED;
+  }
+  if (num_gates == 0) return QSIM_HIP_OK;
+  if (num_gates > kBlockMaxOps) {
+    SetError("a blocked pass holds at most %u gates, got %u", kBlockMaxOps,
+             num_gates);
+    return QSIM_HIP_ERR_ARG;
+  }
+  std::vector<size_t> moff(num_gates);
+  size_t total = 0;
+  for (unsigned i = 0; i < num_gates; ++i) {
+    if (gate_num_qubits[i] < 1 || gate_num_qubits[i] > 4) {
+      SetError("blocked-pass gate %u acts on %u qubits (1..4 supported)", i,
+               gate_num_qubits[i]);
+      return QSIM_HIP_ERR_ARG;
+    }
+    moff[i] = total;
+    total += size_t{2} << (2 * gate_num_qubits[i]);
+  }
+  std::vector<double> m64(total);
+  if (matrix_prec == QSIM_HIP_F64) {
+    std::memcpy(m64.data(), matrices, total * sizeof(double));
+  } else {
+    const float* mf = static_cast<const float*>(matrices);
+    for (size_t i = 0; i < total; ++i) m64[i] = mf[i];
+  }
+  return LaunchGateBlock(static_cast<float2*>(s), n, tile_qubits,
+                         num_tile_qubits, num_gates, gate_num_qubits,
+                         gate_qubits, m64.data(), moff.data());
 }
 
 int qsim_hip_expectation_value(int prec, const void* s, unsigned n,

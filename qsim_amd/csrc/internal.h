@@ -104,6 +104,31 @@ struct TileGeom {
   unsigned low_cvals;
 };
 
+// Cache-blocked passes: a tile of 2^tb amplitudes over the Lc lowest qubits
+// and h = tb - Lc chosen higher qubits is staged in LDS once, and a program
+// of up to kBlockMaxOps 4-qubit gates on tile-local positions runs on it
+// before it is written back.  Tile index -> base = (tile << Lc) with zero
+// bits inserted at the high qubits.
+constexpr unsigned kBlockMaxHigh = 10;
+constexpr unsigned kBlockMaxOps = 32;
+constexpr unsigned kBlockMinBits = 9;   // one 32-group batch of 16 members
+constexpr unsigned kBlockMaxBits = 13;  // 64 KiB of complex64 per tile
+
+struct BlockGeom {
+  uint64_t ntiles;
+  uint64_t hbit[kBlockMaxHigh];
+  unsigned nouter;
+  unsigned outer[kBlockMaxHigh];
+  unsigned Lc;
+  unsigned h;
+  unsigned tb;
+};
+
+struct BlockProgram {
+  unsigned nops;
+  uint8_t tpos[kBlockMaxOps][4];  // tile-local targets of each op, ascending
+};
+
 __host__ __device__ inline uint64_t InsertZeros(uint64_t i, const unsigned* pos,
                                                 unsigned k) {
   for (unsigned j = 0; j < k; ++j) {
@@ -154,5 +179,17 @@ int Convert(void* dst, int dprec, const void* src, int sprec, unsigned n);
 // QSIM_HIP_ERR_UNSUPPORTED when the method cannot handle the gate.
 int LaunchGateMfma(float2* s, unsigned n, const unsigned* qs, unsigned k,
                    const double* m64, int method);
+
+// Split-BF16 A-operand fragments of a 2^K x 2^K complex matrix (FP64,
+// row-major interleaved): 2^K/16 row blocks x 2^(K+1)/16 K blocks x
+// (hi, mid, lo) x 512.
+void BuildFragBf16(int K, const double* m, uint16_t* frag);
+
+// One cache-blocked pass (block_kernels.hip).  `tile` lists the tile qubits
+// ascending; op i acts on ks[i] <= 4 qubits qs[4i ..] (ascending, inside the
+// tile) with the FP64 matrix at m64 + moff[i].
+int LaunchGateBlock(float2* s, unsigned n, const unsigned* tile, unsigned nt,
+                    unsigned nops, const unsigned* ks, const unsigned* qs,
+                    const double* m64, const size_t* moff);
 
 }  // namespace qhip

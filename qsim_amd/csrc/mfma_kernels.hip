@@ -18,9 +18,13 @@
 // 16 rb + 4 s + 2 (l/32) + {0, 1} (rb = row block, s = 0..3): each result is
 // one float2, and 32 lanes store 32 consecutive amplitudes.
 //
-// BF16x3 mode uses v_mfma_f32_32x32x16_bf16 with each FP32 operand split as
-// x = hi + lo (hi = bf16(x), lo = bf16(x - hi)) and A B ~ Ah Bh + Ah Bl + Al Bh
-// accumulated in FP32 (relative error ~2^-16 per product instead of 2^-24).
+// BF16 mode uses v_mfma_f32_32x32x16_bf16 with the matrix split in three
+// bf16 parts A = Ah + Am + Al (relative error 2^-27) and the amplitudes in
+// two, B = Bh + Bl (2^-18), and A B ~ Al Bh + Am Bh + Ah Bl + Ah Bh
+// accumulated in FP32.  The matrix error is the same for every group of a
+// pass, so it adds up coherently (a two-part matrix drifts the norm by up
+// to 2e-6 per pass on structured states); the amplitude error differs from
+// group to group and averages out.  Four MFMAs per K block.
 // Its operand maps: a[e] = A[l%32][8(l/32) + e], b[e] = B[8(l/32) + e][l%32]
 // for e = 0..7 over a K block of 16, i.e. 4 members per lane half.
 //
@@ -38,6 +42,46 @@
 #include "tile.h"
 
 namespace qhip {
+
+// Round-to-nearest-even FP32 -> BF16 bits.
+inline uint16_t Bf16Bits(float x) {
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  u += 0x7fff + ((u >> 16) & 1);
+  return uint16_t(u >> 16);
+}
+inline float Bf16ToFloat(uint16_t b) {
+  uint32_t u = uint32_t(b) << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+// BF16 fragments (hi, mid, lo) per K block kb of 16 reals:
+// frag[((rb * KB + kb) * 3 + part) * 64 * 8 + l * 8 + e] = part of
+// A[32 rb + l%32][16 kb + 8 (l/32) + e].
+void BuildFragBf16(int K, const double* m, uint16_t* frag) {
+  const int D = 1 << K, RB = D / 16, KB = 2 * D / 16;
+  auto A = [&](int i, int k) -> double {
+    int r = i >> 1, ci = i & 1, mm = k >> 1, ck = k & 1;
+    double re = m[2 * (r * D + mm)], im = m[2 * (r * D + mm) + 1];
+    if (ci == 0) return ck == 0 ? re : -im;
+    return ck == 0 ? im : re;
+  };
+  for (int rb = 0; rb < RB; ++rb)
+    for (int kb = 0; kb < KB; ++kb)
+      for (int l = 0; l < 64; ++l)
+        for (int e = 0; e < 8; ++e) {
+          double x = A(32 * rb + (l & 31), 16 * kb + 8 * (l >> 5) + e);
+          const uint16_t hi = Bf16Bits(float(x));
+          const double r1 = x - double(Bf16ToFloat(hi));
+          const uint16_t mid = Bf16Bits(float(r1));
+          const uint16_t lo = Bf16Bits(float(r1 - double(Bf16ToFloat(mid))));
+          frag[((rb * KB + kb) * 3 + 0) * 512 + l * 8 + e] = hi;
+          frag[((rb * KB + kb) * 3 + 1) * 512 + l * 8 + e] = mid;
+          frag[((rb * KB + kb) * 3 + 2) * 512 + l * 8 + e] = lo;
+        }
+}
 
 namespace {
 
@@ -67,43 +111,6 @@ void BuildFragF32(int K, const double* m, float* frag) {
         frag[(rb * D + t) * 64 + l] = float(A(32 * rb + (l & 31), 2 * t + (l >> 5)));
 }
 
-// Round-to-nearest-even FP32 -> BF16 bits.
-inline uint16_t Bf16Bits(float x) {
-  uint32_t u;
-  std::memcpy(&u, &x, 4);
-  u += 0x7fff + ((u >> 16) & 1);
-  return uint16_t(u >> 16);
-}
-inline float Bf16ToFloat(uint16_t b) {
-  uint32_t u = uint32_t(b) << 16;
-  float f;
-  std::memcpy(&f, &u, 4);
-  return f;
-}
-
-// BF16 fragments (hi, lo) per K block kb of 16 reals:
-// frag[((rb * KB + kb) * 2 + part) * 64 * 8 + l * 8 + e] = part of
-// A[32 rb + l%32][16 kb + 8 (l/32) + e].
-void BuildFragBf16(int K, const double* m, uint16_t* frag) {
-  const int D = 1 << K, RB = D / 16, KB = 2 * D / 16;
-  auto A = [&](int i, int k) -> double {
-    int r = i >> 1, ci = i & 1, mm = k >> 1, ck = k & 1;
-    double re = m[2 * (r * D + mm)], im = m[2 * (r * D + mm) + 1];
-    if (ci == 0) return ck == 0 ? re : -im;
-    return ck == 0 ? im : re;
-  };
-  for (int rb = 0; rb < RB; ++rb)
-    for (int kb = 0; kb < KB; ++kb)
-      for (int l = 0; l < 64; ++l)
-        for (int e = 0; e < 8; ++e) {
-          double x = A(32 * rb + (l & 31), 16 * kb + 8 * (l >> 5) + e);
-          uint16_t hi = Bf16Bits(float(x));
-          uint16_t lo = Bf16Bits(float(x - double(Bf16ToFloat(hi))));
-          frag[((rb * KB + kb) * 2 + 0) * 512 + l * 8 + e] = hi;
-          frag[((rb * KB + kb) * 2 + 1) * 512 + l * 8 + e] = lo;
-        }
-}
-
 __device__ __forceinline__ uint64_t MemberOff(unsigned m, const uint64_t* tbit,
                                               int K) {
   uint64_t o = 0;
@@ -119,7 +126,7 @@ __device__ __forceinline__ const T* Launder(const T* p) {
   return p;
 }
 
-// x = hi + lo + O(2^-16 |x|), both parts bf16 (round to nearest even).
+// x = hi + lo + O(2^-18 |x|), both parts bf16 (round to nearest even).
 __device__ __forceinline__ __bf16 Bf16Hi(float x) { return (__bf16)x; }
 __device__ __forceinline__ __bf16 Bf16Lo(float x) {
   return (__bf16)(x - (float)(__bf16)x);
@@ -263,7 +270,7 @@ GateMfmaTileKernel(float2* __restrict__ s, TileGeom g,
 }
 
 // ---------------------------------------------------------------------------
-// BF16x3 MFMA: per K block of 16 reals (8 members), lane half h supplies
+// Split-BF16 MFMA: per K block of 16 reals (8 members), lane half h supplies
 // members 8 kb + 4 h + 0..3 as (re, im) pairs.
 
 template <int K, typename Load, typename Store>
@@ -290,9 +297,11 @@ __device__ __forceinline__ void BatchBf16(const uint16_t* __restrict__ frag,
     }
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb) {
-      const v8bf ah = f[((rb * KB + kb) * 2 + 0) * 64 + l];
-      const v8bf al = f[((rb * KB + kb) * 2 + 1) * 64 + l];
+      const v8bf ah = f[((rb * KB + kb) * 3 + 0) * 64 + l];
+      const v8bf am = f[((rb * KB + kb) * 3 + 1) * 64 + l];
+      const v8bf al = f[((rb * KB + kb) * 3 + 2) * 64 + l];
       c[rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, c[rb], 0, 0, 0);
+      c[rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(am, bh, c[rb], 0, 0, 0);
       c[rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, c[rb], 0, 0, 0);
       c[rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, c[rb], 0, 0, 0);
     }
@@ -394,7 +403,7 @@ int DispatchMfma(float2* s, unsigned n, const unsigned* qs, const double* m64,
     BuildFragF32(K, m64, f32.data());
     fbytes = f32.size() * sizeof(float);
   } else {
-    fb.resize(size_t(Dims<K>::RB) * (2 * D / 16) * 2 * 512);
+    fb.resize(size_t(Dims<K>::RB) * (2 * D / 16) * 3 * 512);
     BuildFragBf16(K, m64, fb.data());
     fbytes = fb.size() * sizeof(uint16_t);
   }

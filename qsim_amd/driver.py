@@ -20,6 +20,11 @@ DRIVER_PATH = os.path.join(HERE, "lib", "libqsim_hip_driver.so")
 FUSER_MQUBIT = 0
 FUSER_LEARNED = 1
 
+PASS_GATE = 0
+PASS_MEASUREMENT = 1
+PASS_OPERATION = 2
+PASS_BLOCKED = 3
+
 _dll = None
 
 
@@ -40,8 +45,12 @@ def driver_lib():
                 ("qsim_driver_num_qubits", u, [vp]),
                 ("qsim_driver_num_gates", u, [vp]),
                 ("qsim_driver_fuse", i, [vp, u, i, i]),
+                ("qsim_driver_fuse_blocked", i, [vp, u, u]),
                 ("qsim_driver_num_passes", u, [vp]),
+                ("qsim_driver_pass_kind", i, [vp, u]),
                 ("qsim_driver_pass_info", i, [vp, u, up, up, up]),
+                ("qsim_driver_block_info", i, [vp, u, up, up, up,
+                                               ctypes.POINTER(ctypes.c_double)]),
                 ("qsim_driver_run_passes", i, [vp, vp, i, u, u, u]),
                 ("qsim_driver_run", i, [vp, vp, i, u])):
             f = getattr(d, name)
@@ -66,6 +75,35 @@ class Circuit:
         _check(driver_lib().qsim_driver_fuse(self.h, max_fused_size, fuser,
                                              F64 if f64_matrices else F32))
         return self.num_passes
+
+    def fuse_blocked(self, tile_bits: int = 13, low_bits: int = 5):
+        """Cache-blocked passes (FP64 fused matrices, BF16x3 MFMA)."""
+        _check(driver_lib().qsim_driver_fuse_blocked(self.h, tile_bits,
+                                                     low_bits))
+        return self.num_passes
+
+    def pass_kind(self, i: int) -> int:
+        r = driver_lib().qsim_driver_pass_kind(self.h, i)
+        if r < 0:
+            _check(r)
+        return r
+
+    def block_info(self, i: int):
+        """[(qubits, FP64 interleaved matrix), ...]: the fused gates of
+        blocked pass i."""
+        n = ctypes.c_uint32(0)
+        ks = (ctypes.c_uint32 * 64)()
+        qs = (ctypes.c_uint32 * 256)()
+        ms = np.zeros(32 * 512, np.float64)
+        _check(driver_lib().qsim_driver_block_info(
+            self.h, i, ctypes.byref(n), ks, qs,
+            ms.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        out, off = [], 0
+        for g in range(n.value):
+            size = 2 << (2 * ks[g])
+            out.append((list(qs[4 * g:4 * g + ks[g]]), ms[off:off + size].copy()))
+            off += size
+        return out
 
     @property
     def num_passes(self) -> int:

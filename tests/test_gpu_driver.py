@@ -109,3 +109,53 @@ def test_measurement_circuit_matches_oracle():
         ss.set_state_zero(st)
         c.run(st, seed=seed)
         assert rel_l2(st.to_numpy(), s_or) < 1e-12, seed
+
+
+@pytest.mark.parametrize("tile_bits,low_bits", [(13, 5), (12, 6), (10, 3)])
+def test_grid20_blocked_vs_oracle(grid20, tile_bits, low_bits):
+    # Cache-blocked passes (BF16x3 MFMA, FP64 fused matrices): the state
+    # meets F >= 1 - 1e-6 against the FP64 oracle and stays within 1e-4
+    # relative L2 of it.
+    text, s32, s64 = grid20
+    c = driver.Circuit(text)
+    npass = c.fuse_blocked(tile_bits, low_bits)
+    ss = qsim_amd.StateSpace(np.float32)
+    st = ss.create(20)
+    ss.set_state_zero(st)
+    c.run(st)
+    got = st.to_numpy()
+    fid = abs(np.vdot(from_interleaved(s64), from_interleaved(got))) ** 2
+    err = rel_l2(got, s64)
+    print(f"blocked {tile_bits}/{low_bits}: {npass} passes, rel-L2 vs FP64 "
+          f"{err:.2e}, 1-F {1 - fid:.2e}", flush=True)
+    assert err < 1e-4
+    assert fid > 1 - 1e-6
+    # The complex128 replay of the same partition is exact.
+    ss64 = qsim_amd.StateSpace(np.float64)
+    st64 = ss64.create(20)
+    ss64.set_state_zero(st64)
+    c.run(st64)
+    assert rel_l2(st64.to_numpy(), s64) < 1e-12
+
+
+def test_blocked_random_circuit_with_controls_and_measurement():
+    # Controlled gates and measurements stay single passes between blocked
+    # ones; the result equals the oracle's (same measurement draws).
+    text = circuits.random_circuit(
+        16, 400, seed=11, controlled_fraction=0.05,
+        gate_names=("h", "t", "rx", "ry", "rz", "cz", "cnot", "fs", "is"))
+    lines = text.strip().split("\n")
+    lines.insert(200, "199 m 3 7 11")
+    text = "\n".join(lines) + "\n"
+    want, mres = oracle.Circuit(text, np.float64).run(seed=3)
+    c = driver.Circuit(text)
+    c.fuse_blocked(12, 5)
+    kinds = [c.pass_kind(i) for i in range(c.num_passes)]
+    assert driver.PASS_MEASUREMENT in kinds and driver.PASS_OPERATION in kinds
+    ss = qsim_amd.StateSpace(np.float32)
+    st = ss.create(16)
+    ss.set_state_zero(st)
+    c.run(st, seed=3)
+    got = st.to_numpy()
+    fid = abs(np.vdot(from_interleaved(want), from_interleaved(got))) ** 2
+    assert fid > 1 - 1e-6, fid
