@@ -215,6 +215,19 @@ int qsim_hip_apply_gate_block(int prec, void* state, unsigned num_qubits,
                               const unsigned* gate_qubits,
                               const void* matrices, int matrix_prec);
 
+/* The same pass prepared once for n-qubit states: validation, tile geometry
+ * and the gates' MFMA operand fragments are built and uploaded at prepare
+ * time, so a run is one kernel launch.  A handle is not thread-safe;
+ * qsim_hip_block_free waits for launches that may still read it. */
+int qsim_hip_block_prepare(unsigned num_qubits, const unsigned* tile_qubits,
+                           unsigned num_tile_qubits, unsigned num_gates,
+                           const unsigned* gate_num_qubits,
+                           const unsigned* gate_qubits, const void* matrices,
+                           int matrix_prec, void** handle);
+int qsim_hip_block_run(void* handle, int prec, void* state,
+                       unsigned num_qubits);
+int qsim_hip_block_free(void* handle);
+
 /* <psi|U|psi> for k <= 6 with FP64 accumulation.  Replaces SimulatorCUDA::
  * ExpectationValue (lib/simulator_cuda.h:216-260). */
 int qsim_hip_expectation_value(int prec, const void* state,
@@ -244,6 +257,37 @@ int qsim_hip_event_elapsed_ms(int slot_start, int slot_end, float* ms);
  * (lib/run_qsim.h:129-142). */
 int qsim_hip_pass_timer_start(unsigned capacity);
 int qsim_hip_pass_timer_stop(float* ms, unsigned max_count, unsigned* count);
+
+/* ---- pass-time cost model ------------------------------------------------
+ * A multilayer perceptron num_features -> hidden -> hidden -> hidden -> 1
+ * (ReLU hidden layers) that predicts a pass's log time from features of the
+ * pass; trained and evaluated on the device by the library's own kernels
+ * (FP32, Adam, mean squared error).  Parameters are one flat array, layer by
+ * layer: W (in x out, row-major), then b.  No reference counterpart: qsim's
+ * MultiQubitGateFuser (lib/fuser_mqubit.h:185-526) picks its partition from
+ * max_fused_size alone; the planner scores candidate partitions with this
+ * model.  Not thread-safe per model. */
+int qsim_hip_costmodel_create(unsigned num_features, unsigned hidden,
+                              void** model);
+int qsim_hip_costmodel_free(void* model);
+uint64_t qsim_hip_costmodel_num_params(void* model);
+int qsim_hip_costmodel_set_params(void* model, const float* params,
+                                  uint64_t num_params);
+int qsim_hip_costmodel_get_params(void* model, float* params,
+                                  uint64_t num_params);
+/* y[i] = network(x[i * num_features ...]) for i < batch (host arrays). */
+int qsim_hip_costmodel_forward(void* model, const float* x, unsigned batch,
+                               float* y);
+/* Mean squared error of the batch against targets t and its gradient with
+ * respect to the parameters (host arrays; grad may be NULL). */
+int qsim_hip_costmodel_gradient(void* model, const float* x, const float* t,
+                                unsigned batch, double* loss, float* grad);
+/* Adam training over num_samples samples for `epochs` epochs of seeded
+ * shuffled minibatches; final_loss = mean squared error over the set. */
+int qsim_hip_costmodel_train(void* model, const float* x, const float* t,
+                             unsigned num_samples, unsigned epochs,
+                             unsigned batch, float lr, unsigned seed,
+                             double* final_loss);
 
 #ifdef __cplusplus
 }  /* extern "C" */

@@ -18,7 +18,7 @@ extern "C" {
 /* Fuser selection for qsim_driver_fuse. */
 enum {
   QSIM_DRIVER_FUSER_MQUBIT = 0,  /* qsim's MultiQubitGateFuser */
-  QSIM_DRIVER_FUSER_LEARNED = 1  /* cost-model fuser (qsim_amd/fuser_learned.h) */
+  QSIM_DRIVER_FUSER_LEARNED = 1  /* cost-model planner (qsim_driver_plan_learned) */
 };
 
 /* Parses qsim's circuit text format up to `maxtime` (CircuitQsimParser::
@@ -68,6 +68,37 @@ int qsim_driver_pass_info(void* circuit, unsigned i, unsigned* k,
 int qsim_driver_block_info(void* circuit, unsigned i, unsigned* num_ops,
                            unsigned* op_k, unsigned* op_qubits,
                            double* op_matrices);
+
+/* Gate method the current partition runs with (QSIM_HIP_METHOD_*, or -1 for
+ * the library's current one); set for the run only and restored after. */
+int qsim_driver_set_method(void* circuit, int method);
+
+/* Cost-model features (include/qsim_amd/learned_planner.h, 18 floats) of
+ * pass i of the current partition when run with gate method `method`. */
+int qsim_driver_pass_features(void* circuit, unsigned i, int method,
+                              float* out);
+
+/* The learned planner: builds the candidate partitions of
+ * qsim::hip::DefaultCandidates(), predicts each pass's time with the cost
+ * model (qsim_hip_costmodel_*, log ms per 2^24 amplitudes), and leaves the
+ * circuit partitioned by the candidate with the least predicted total.
+ * model == NULL selects the fallback, qsim's fuser at max_fused_size 4.
+ * Per candidate (up to max_candidates entries, any pointer may be NULL):
+ * blocked flag, a (max_fused_size or tile qubits), b (low tile qubits),
+ * gate method, predicted ms (NaN without a model), pass count.  *chosen is
+ * the index of the candidate in use.  No reference counterpart: it replaces
+ * the fixed max_fused_size of MultiQubitGateFuser::FuseGates
+ * (lib/fuser_mqubit.h:185-526) as called by QSimRunner::Run
+ * (lib/run_qsim.h:61-289). */
+int qsim_driver_plan_learned(void* circuit, void* model,
+                             unsigned max_candidates, int* blocked,
+                             unsigned* a, unsigned* b, int* methods,
+                             double* predicted_ms, unsigned* num_passes,
+                             unsigned* num_candidates, int* chosen);
+
+/* Model used by qsim_driver_fuse(..., QSIM_DRIVER_FUSER_LEARNED, ...)
+ * (NULL: the fallback partition). */
+void qsim_driver_set_cost_model(void* model);
 
 /* Applies passes [first, first + count) to a state of the circuit's size
  * (the apply loop of QSimRunner::Run, lib/run_qsim.h:264-280); measurement

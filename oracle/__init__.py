@@ -79,7 +79,40 @@ def lib():
             ctypes.c_uint64, ctypes.c_uint, ctypes.c_double, _f64p]
         _lib.orc_num_threads.restype = ctypes.c_int
         _lib.orc_set_num_threads.argtypes = [ctypes.c_int]
+        _lib.orc_mlp_num_params.argtypes = [ctypes.c_uint, ctypes.c_uint]
+        _lib.orc_mlp_num_params.restype = ctypes.c_ulong
+        _lib.orc_mlp_forward.argtypes = [ctypes.c_uint, ctypes.c_uint, _f32p,
+                                         _f32p, ctypes.c_uint, _f64p]
+        _lib.orc_mlp_gradient.argtypes = [ctypes.c_uint, ctypes.c_uint, _f32p,
+                                          _f32p, _f32p, ctypes.c_uint, _f64p]
+        _lib.orc_mlp_gradient.restype = ctypes.c_double
     return _lib
+
+
+def mlp_num_params(num_features, hidden):
+    """Parameter count of the cost-model network (mlp_oracle.cc)."""
+    return int(lib().orc_mlp_num_params(num_features, hidden))
+
+
+def mlp_forward(num_features, hidden, params, x):
+    """FP64 forward pass of the cost-model network on rows of x."""
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    xx = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, num_features)
+    y = np.zeros(len(xx), np.float64)
+    lib().orc_mlp_forward(num_features, hidden, _ptr(p), _ptr(xx), len(xx),
+                          _ptr(y))
+    return y
+
+
+def mlp_gradient(num_features, hidden, params, x, t):
+    """(mean squared error, its FP64 gradient w.r.t. the parameters)."""
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    xx = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, num_features)
+    tt = np.ascontiguousarray(t, dtype=np.float32)
+    g = np.zeros(len(p), np.float64)
+    loss = lib().orc_mlp_gradient(num_features, hidden, _ptr(p), _ptr(xx),
+                                  _ptr(tt), len(xx), _ptr(g))
+    return float(loss), g
 
 
 def _suf(dtype):

@@ -87,6 +87,19 @@ SIGNATURES = [
     ("qsim_hip_event_elapsed_ms", _i, [_i, _i, ctypes.POINTER(ctypes.c_float)]),
     ("qsim_hip_pass_timer_start", _i, [_u]),
     ("qsim_hip_pass_timer_stop", _i, [ctypes.POINTER(ctypes.c_float), _u, _u32p]),
+    ("qsim_hip_block_prepare", _i, [_u, _u32p, _u, _u, _u32p, _u32p, _vp, _i,
+                                    ctypes.POINTER(_vp)]),
+    ("qsim_hip_block_run", _i, [_vp, _i, _vp, _u]),
+    ("qsim_hip_block_free", _i, [_vp]),
+    ("qsim_hip_costmodel_create", _i, [_u, _u, ctypes.POINTER(_vp)]),
+    ("qsim_hip_costmodel_free", _i, [_vp]),
+    ("qsim_hip_costmodel_num_params", _u64, [_vp]),
+    ("qsim_hip_costmodel_set_params", _i, [_vp, _vp, _u64]),
+    ("qsim_hip_costmodel_get_params", _i, [_vp, _vp, _u64]),
+    ("qsim_hip_costmodel_forward", _i, [_vp, _vp, _u, _vp]),
+    ("qsim_hip_costmodel_gradient", _i, [_vp, _vp, _vp, _u, _f64p, _vp]),
+    ("qsim_hip_costmodel_train", _i, [_vp, _vp, _vp, _u, _u, _u,
+                                      ctypes.c_float, _u, _f64p]),
 ]
 
 
@@ -384,6 +397,51 @@ class StateSpace:
         return res
 
 
+def _block_args(tile_qubits, gates):
+    tarr, tp = _u32(sorted(tile_qubits))
+    ks, qs, ms = [], [], []
+    for q, m in gates:
+        q = list(q)
+        ks.append(len(q))
+        qs.extend(q + [0] * (4 - len(q)))
+        ms.append(np.asarray(m, dtype=np.float64).reshape(-1))
+    karr, kp = _u32(ks)
+    qarr, qp = _u32(qs)
+    mat = (np.ascontiguousarray(np.concatenate(ms)) if ms
+           else np.zeros(2, np.float64))
+    return (tarr, tp, karr, kp, qarr, qp, mat)
+
+
+class PreparedBlock:
+    """A cache-blocked pass whose fragments live on the device
+    (qsim_hip_block_prepare / _run / _free)."""
+
+    def __init__(self, num_qubits, tile_qubits, gates):
+        a = _block_args(tile_qubits, gates)
+        tarr, tp, karr, kp, qarr, qp, mat = a
+        h = ctypes.c_void_p()
+        _check(lib().qsim_hip_block_prepare(num_qubits, tp, len(tarr),
+                                            len(karr), kp, qp,
+                                            mat.ctypes.data, F64,
+                                            ctypes.byref(h)))
+        self.h = h
+        self.num_qubits = num_qubits
+
+    def run(self, s: "State"):
+        _check(lib().qsim_hip_block_run(self.h, s.prec, s.ptr, s.num_qubits))
+
+    def free(self):
+        if getattr(self, "h", None):
+            _check(lib().qsim_hip_block_free(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class Simulator:
     """Python mirror of SimulatorCUDA (lib/simulator_cuda.h:34-260)."""
 
@@ -420,6 +478,10 @@ class Simulator:
         _check(lib().qsim_hip_apply_gate_block(
             s.prec, s.ptr, s.num_qubits, tp, len(tarr), len(ks), kp, qp,
             mat.ctypes.data, F64))
+
+    def prepare_gate_block(self, num_qubits, tile_qubits, gates):
+        """A blocked pass prepared once (qsim_hip_block_prepare)."""
+        return PreparedBlock(num_qubits, tile_qubits, gates)
 
     def expectation_value(self, qs, matrix, s: State) -> complex:
         qarr, qp = _u32(qs)

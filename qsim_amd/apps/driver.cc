@@ -21,6 +21,7 @@
 #include "gate_appl.h"
 #include "io.h"
 #include "qsim_amd/block_planner.h"
+#include "qsim_amd/learned_planner.h"
 #include "qsim_amd/simulator_hip.h"
 #include "qsim_hip.h"
 #include "qsim_hip_driver.h"
@@ -28,6 +29,9 @@
 namespace {
 
 using namespace qsim;
+
+// Cost model used by qsim_driver_fuse(..., QSIM_DRIVER_FUSER_LEARNED, ...).
+void* g_cost_model = nullptr;
 
 template <typename FP>
 using FusedOps =
@@ -48,9 +52,21 @@ struct DriverCircuit {
   FusedOps<double> f64;
   std::vector<hip::BlockPass> blocks;
   std::vector<Step> steps;
+  // Gate method the partition runs with (-1: the library's current one).
+  int method = -1;
+  // Prepared blocked passes (qsim_hip_block_prepare), one per block, made at
+  // the first run.
+  std::vector<void*> prepared;
+
+  void ReleasePrepared() {
+    for (void* h : prepared) qsim_hip_block_free(h);
+    prepared.clear();
+  }
+  ~DriverCircuit() { ReleasePrepared(); }
 };
 
 void PlainSteps(DriverCircuit& d) {
+  d.ReleasePrepared();
   d.blocks.clear();
   d.steps.clear();
   for (size_t i = 0; i < d.f32.size(); ++i) d.steps.push_back({false, i});
@@ -67,6 +83,7 @@ template <typename FP>
 int Fuse(const Circuit<Operation<FP>>& c, unsigned max_fused, int fuser,
          FusedOps<FP>& out) {
   if (fuser == QSIM_DRIVER_FUSER_MQUBIT) {
+    if (max_fused < 1 || max_fused > 6) return QSIM_HIP_ERR_ARG;
     MultiQubitGateFuser<IO>::Parameter p;
     p.max_fused_size = max_fused;
     p.verbosity = 0;
@@ -98,10 +115,22 @@ void PassInfo(const FusedOp& op, unsigned* k, unsigned* qubits,
   if (num_gates) *num_gates = ng;
 }
 
-int RunBlock(const hip::BlockPass& b, float* p, unsigned n) {
-  return qsim_hip_apply_gate_block(QSIM_HIP_F32, p, n, b.tile.data(),
-                                   b.tile.size(), b.ks.size(), b.ks.data(),
-                                   b.qs.data(), b.m.data(), QSIM_HIP_F64);
+// Blocked pass `index`: prepared (fragments on the device) at its first run,
+// one kernel launch per run after that.
+int RunBlock(DriverCircuit& d, size_t index, float* p, unsigned n) {
+  if (d.prepared.size() != d.blocks.size()) {
+    d.ReleasePrepared();
+    d.prepared.assign(d.blocks.size(), nullptr);
+  }
+  if (d.prepared[index] == nullptr) {
+    const hip::BlockPass& b = d.blocks[index];
+    int rc = qsim_hip_block_prepare(n, b.tile.data(), b.tile.size(),
+                                    b.ks.size(), b.ks.data(), b.qs.data(),
+                                    b.m.data(), QSIM_HIP_F64,
+                                    &d.prepared[index]);
+    if (rc != QSIM_HIP_OK) return rc;
+  }
+  return qsim_hip_block_run(d.prepared[index], QSIM_HIP_F32, p, n);
 }
 
 // Complex64 state, FP32 matrices: exactly QSimRunner's loop.
@@ -114,7 +143,7 @@ int RunF32(DriverCircuit& d, float* p, unsigned first, unsigned count,
   for (unsigned i = first; i < first + count; ++i) {
     const Step& st = d.steps[i];
     if (st.blocked) {
-      int rc = RunBlock(d.blocks[st.index], p, d.c32.num_qubits);
+      int rc = RunBlock(d, st.index, p, d.c32.num_qubits);
       if (rc != QSIM_HIP_OK) return rc;
     } else if (!ApplyGate(ss, sim, d.f32[st.index], rgen, state)) {
       return QSIM_HIP_ERR_ARG;
@@ -134,7 +163,7 @@ int RunF32M64(DriverCircuit& d, float* p, unsigned first, unsigned count,
   for (unsigned i = first; i < first + count; ++i) {
     const Step& st = d.steps[i];
     if (st.blocked) {
-      int rc = RunBlock(d.blocks[st.index], p, n);
+      int rc = RunBlock(d, st.index, p, n);
       if (rc != QSIM_HIP_OK) return rc;
       continue;
     }
@@ -222,6 +251,56 @@ int PlanBlocked(const Circuit<Operation<FP>>& c, unsigned tile_bits,
   return QSIM_HIP_OK;
 }
 
+bool IsDiagonal(const std::vector<float>& m, unsigned k) {
+  const unsigned D = 1u << k;
+  for (unsigned r = 0; r < D; ++r)
+    for (unsigned c = 0; c < D; ++c)
+      if (r != c && (m[2 * (r * D + c)] != 0 || m[2 * (r * D + c) + 1] != 0))
+        return false;
+  return true;
+}
+
+// Cost-model features of step i when run with gate method `method`.
+void PassFeatures(const DriverCircuit& d, unsigned i, int method, float* f) {
+  const unsigned n = d.c32.num_qubits;
+  const Step& st = d.steps[i];
+  if (st.blocked) {
+    const hip::BlockPass& b = d.blocks[st.index];
+    hip::BlockFeatures(n, b.tile, b.ks.size(), f);
+    return;
+  }
+  const auto& op = d.f32[st.index];
+  if (const auto* fg = std::get_if<FusedGate<float>>(&op)) {
+    const unsigned k = fg->qubits.size();
+    hip::GateFeatures(n, fg->qubits, 0, IsDiagonal(fg->matrix, k),
+                      hip::GateFamily(k, 0, method, n), f);
+    return;
+  }
+  const Operation<float>* o = nullptr;
+  if (const auto* pp = std::get_if<const Operation<float>*>(&op)) o = *pp;
+  if (o != nullptr) {
+    if (const auto* cg = OpGetAlternative<ControlledGate<float>>(*o)) {
+      hip::GateFeatures(n, cg->qubits, cg->controlled_by.size(), false,
+                        hip::kVector, f);
+      return;
+    }
+  }
+  // Measurements: one reduction pass over the measured qubits.
+  const std::vector<unsigned>* qs = nullptr;
+  if (const auto* m = std::get_if<Measurement>(&op)) qs = &m->qubits;
+  else if (o != nullptr) qs = &OpQubits(*o);
+  static const std::vector<unsigned> none;
+  hip::GateFeatures(n, qs ? *qs : none, 0, false, hip::kVector, f);
+}
+
+int ApplyCandidate(DriverCircuit& d, const hip::PlanCandidate& c) {
+  int rc = c.blocked ? qsim_driver_fuse_blocked(&d, c.a, c.b)
+                     : qsim_driver_fuse(&d, c.a, QSIM_DRIVER_FUSER_MQUBIT,
+                                        QSIM_HIP_F64);
+  d.method = rc == QSIM_HIP_OK ? c.method : -1;
+  return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -244,9 +323,17 @@ unsigned qsim_driver_num_gates(void* c) {
   return static_cast<DriverCircuit*>(c)->c32.ops.size();
 }
 
+void qsim_driver_set_cost_model(void* model) { g_cost_model = model; }
+
 int qsim_driver_fuse(void* c, unsigned max_fused_size, int fuser,
                      int matrix_prec) {
   auto* d = static_cast<DriverCircuit*>(c);
+  d->method = -1;
+  if (fuser == QSIM_DRIVER_FUSER_LEARNED) {
+    return qsim_driver_plan_learned(c, g_cost_model, 0, nullptr, nullptr,
+                                    nullptr, nullptr, nullptr, nullptr,
+                                    nullptr, nullptr);
+  }
   d->matrix_prec = matrix_prec;
   int rc = Fuse(d->c32, max_fused_size, fuser, d->f32);
   if (rc == QSIM_HIP_OK) rc = Fuse(d->c64, max_fused_size, fuser, d->f64);
@@ -264,7 +351,9 @@ int qsim_driver_fuse_blocked(void* c, unsigned tile_bits, unsigned low_bits) {
       low_bits > tile_bits) {
     return QSIM_HIP_ERR_ARG;
   }
+  d->method = -1;
   d->matrix_prec = QSIM_HIP_F64;
+  d->ReleasePrepared();
   std::vector<hip::BlockPass> b32;
   std::vector<Step> s32;
   int rc = PlanBlocked(d->c64, tile_bits, low_bits, d->blocks, d->steps,
@@ -347,10 +436,101 @@ int qsim_driver_run_passes(void* c, void* state, int prec, unsigned first,
   if (prec == QSIM_HIP_F64) {
     return RunF64(*d, static_cast<double*>(state), first, count, seed);
   }
-  if (d->matrix_prec == QSIM_HIP_F64) {
-    return RunF32M64(*d, static_cast<float*>(state), first, count, seed);
+  const int old_method = qsim_hip_get_gate_method();
+  if (d->method >= 0 && d->method != old_method) {
+    int rc = qsim_hip_set_gate_method(d->method);
+    if (rc != QSIM_HIP_OK) return rc;
   }
-  return RunF32(*d, static_cast<float*>(state), first, count, seed);
+  int rc = d->matrix_prec == QSIM_HIP_F64
+               ? RunF32M64(*d, static_cast<float*>(state), first, count, seed)
+               : RunF32(*d, static_cast<float*>(state), first, count, seed);
+  if (d->method >= 0 && d->method != old_method) {
+    qsim_hip_set_gate_method(old_method);
+  }
+  return rc;
+}
+
+int qsim_driver_set_method(void* c, int method) {
+  auto* d = static_cast<DriverCircuit*>(c);
+  if (method < -1 || method > QSIM_HIP_METHOD_MFMA_BF16X3) {
+    return QSIM_HIP_ERR_ARG;
+  }
+  d->method = method;
+  return QSIM_HIP_OK;
+}
+
+int qsim_driver_pass_features(void* c, unsigned i, int method, float* out) {
+  auto* d = static_cast<DriverCircuit*>(c);
+  if (i >= d->steps.size() || out == nullptr) return QSIM_HIP_ERR_ARG;
+  PassFeatures(*d, i, method, out);
+  return QSIM_HIP_OK;
+}
+
+int qsim_driver_plan_learned(void* c, void* model, unsigned max_candidates,
+                             int* blocked, unsigned* a, unsigned* b,
+                             int* methods, double* predicted_ms,
+                             unsigned* num_passes, unsigned* num_candidates,
+                             int* chosen) {
+  auto* d = static_cast<DriverCircuit*>(c);
+  const unsigned n = d->c64.num_qubits;
+  std::vector<hip::PlanCandidate> cands = hip::DefaultCandidates();
+  std::vector<float> feats;
+  std::vector<unsigned> owner;
+  std::vector<int> ok(cands.size(), 0);
+  std::vector<unsigned> npass(cands.size(), 0);
+  if (model != nullptr) {
+    for (size_t j = 0; j < cands.size(); ++j) {
+      if (ApplyCandidate(*d, cands[j]) != QSIM_HIP_OK) continue;
+      ok[j] = 1;
+      npass[j] = d->steps.size();
+      for (unsigned i = 0; i < d->steps.size(); ++i) {
+        const size_t o = feats.size();
+        feats.resize(o + hip::kCostFeatures);
+        PassFeatures(*d, i, cands[j].method, &feats[o]);
+        owner.push_back(j);
+      }
+    }
+  }
+  std::vector<double> pred(cands.size(), std::nan(""));
+  int best = -1;
+  if (!owner.empty()) {
+    std::vector<float> y(owner.size());
+    int rc = qsim_hip_costmodel_forward(model, feats.data(), owner.size(),
+                                        y.data());
+    if (rc != QSIM_HIP_OK) return rc;
+    for (size_t j = 0; j < cands.size(); ++j) pred[j] = ok[j] ? 0.0 : pred[j];
+    for (size_t r = 0; r < owner.size(); ++r) {
+      pred[owner[r]] += std::exp(double(y[r]) + hip::CostTargetOffset(n));
+    }
+    for (size_t j = 0; j < cands.size(); ++j) {
+      if (ok[j] && (best < 0 || pred[j] < pred[best])) best = int(j);
+    }
+  }
+  hip::PlanCandidate pick =
+      best >= 0 ? cands[best] : hip::FallbackCandidate();
+  int rc = ApplyCandidate(*d, pick);
+  if (rc != QSIM_HIP_OK) return rc;
+  if (best < 0) {
+    // No model: the fallback, reported by its index in the list.
+    for (size_t j = 0; j < cands.size(); ++j) {
+      if (!cands[j].blocked && cands[j].a == pick.a &&
+          cands[j].method == pick.method) {
+        best = int(j);
+      }
+    }
+  }
+  const unsigned m = std::min<unsigned>(max_candidates, cands.size());
+  for (unsigned j = 0; j < m; ++j) {
+    if (blocked) blocked[j] = cands[j].blocked;
+    if (a) a[j] = cands[j].a;
+    if (b) b[j] = cands[j].b;
+    if (methods) methods[j] = cands[j].method;
+    if (predicted_ms) predicted_ms[j] = pred[j];
+    if (num_passes) num_passes[j] = npass[j];
+  }
+  if (num_candidates) *num_candidates = cands.size();
+  if (chosen) *chosen = best;
+  return QSIM_HIP_OK;
 }
 
 int qsim_driver_run(void* c, void* state, int prec, unsigned seed) {

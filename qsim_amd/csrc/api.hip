@@ -194,6 +194,36 @@ int CheckState(int prec, const void* s, unsigned n) {
 
 uint64_t ElemBytes(int prec) { return prec == QSIM_HIP_F32 ? 4 : 8; }
 
+// Offsets and FP64 copies of a blocked pass's concatenated gate matrices.
+int BlockMatrices(unsigned num_gates, const unsigned* ks, const void* matrices,
+                  int matrix_prec, std::vector<size_t>& moff,
+                  std::vector<double>& m64) {
+  if (num_gates > kBlockMaxOps) {
+    SetError("a blocked pass holds at most %u gates, got %u", kBlockMaxOps,
+             num_gates);
+    return QSIM_HIP_ERR_ARG;
+  }
+  moff.resize(num_gates);
+  size_t total = 0;
+  for (unsigned i = 0; i < num_gates; ++i) {
+    if (ks[i] < 1 || ks[i] > 4) {
+      SetError("blocked-pass gate %u acts on %u qubits (1..4 supported)", i,
+               ks[i]);
+      return QSIM_HIP_ERR_ARG;
+    }
+    moff[i] = total;
+    total += size_t{2} << (2 * ks[i]);
+  }
+  m64.resize(total);
+  if (matrix_prec == QSIM_HIP_F64) {
+    std::memcpy(m64.data(), matrices, total * sizeof(double));
+  } else {
+    const float* mf = static_cast<const float*>(matrices);
+    for (size_t i = 0; i < total; ++i) m64[i] = mf[i];
+  }
+  return QSIM_HIP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -537,32 +567,53 @@ int qsim_hip_apply_gate_block(int prec, void* s, unsigned n,
     return QSIM_HIP_ERR_UNSUPPORTED;
   }
   if (num_gates == 0) return QSIM_HIP_OK;
-  if (num_gates > kBlockMaxOps) {
-    SetError("a blocked pass holds at most %u gates, got %u", kBlockMaxOps,
-             num_gates);
-    return QSIM_HIP_ERR_ARG;
-  }
-  std::vector<size_t> moff(num_gates);
-  size_t total = 0;
-  for (unsigned i = 0; i < num_gates; ++i) {
-    if (gate_num_qubits[i] < 1 || gate_num_qubits[i] > 4) {
-      SetError("blocked-pass gate %u acts on %u qubits (1..4 supported)", i,
-               gate_num_qubits[i]);
-      return QSIM_HIP_ERR_ARG;
-    }
-    moff[i] = total;
-    total += size_t{2} << (2 * gate_num_qubits[i]);
-  }
-  std::vector<double> m64(total);
-  if (matrix_prec == QSIM_HIP_F64) {
-    std::memcpy(m64.data(), matrices, total * sizeof(double));
-  } else {
-    const float* mf = static_cast<const float*>(matrices);
-    for (size_t i = 0; i < total; ++i) m64[i] = mf[i];
-  }
+  std::vector<size_t> moff;
+  std::vector<double> m64;
+  QHIP_TRY(BlockMatrices(num_gates, gate_num_qubits, matrices, matrix_prec,
+                         moff, m64));
   return LaunchGateBlock(static_cast<float2*>(s), n, tile_qubits,
                          num_tile_qubits, num_gates, gate_num_qubits,
                          gate_qubits, m64.data(), moff.data());
+}
+
+int qsim_hip_block_prepare(unsigned n, const unsigned* tile_qubits,
+                           unsigned num_tile_qubits, unsigned num_gates,
+                           const unsigned* gate_num_qubits,
+                           const unsigned* gate_qubits, const void* matrices,
+                           int matrix_prec, void** handle) {
+  if (handle == nullptr || tile_qubits == nullptr || num_gates == 0 ||
+      gate_num_qubits == nullptr || gate_qubits == nullptr ||
+      matrices == nullptr) {
+    SetError("null or empty blocked-pass operand");
+    return QSIM_HIP_ERR_ARG;
+  }
+  *handle = nullptr;
+  QHIP_TRY(EnsureInit());
+  std::vector<size_t> moff;
+  std::vector<double> m64;
+  QHIP_TRY(BlockMatrices(num_gates, gate_num_qubits, matrices, matrix_prec,
+                         moff, m64));
+  return PrepareBlockHandle(n, tile_qubits, num_tile_qubits, num_gates,
+                            gate_num_qubits, gate_qubits, m64.data(),
+                            moff.data(), handle);
+}
+
+int qsim_hip_block_run(void* handle, int prec, void* s, unsigned n) {
+  QHIP_TRY(CheckState(prec, s, n));
+  if (handle == nullptr) {
+    SetError("null blocked-pass handle");
+    return QSIM_HIP_ERR_ARG;
+  }
+  if (prec != QSIM_HIP_F32) {
+    SetError("blocked passes run on complex64 states only");
+    return QSIM_HIP_ERR_UNSUPPORTED;
+  }
+  return RunBlockHandle(handle, static_cast<float2*>(s), n);
+}
+
+int qsim_hip_block_free(void* handle) {
+  if (handle == nullptr) return QSIM_HIP_OK;
+  return FreeBlockHandle(handle);
 }
 
 int qsim_hip_expectation_value(int prec, const void* s, unsigned n,

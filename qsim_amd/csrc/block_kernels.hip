@@ -158,9 +158,10 @@ void PadTo4(const double* u, unsigned k, const unsigned* pos,
 
 }  // namespace
 
-int LaunchGateBlock(float2* s, unsigned n, const unsigned* tile, unsigned nt,
-                    unsigned nops, const unsigned* ks, const unsigned* qs,
-                    const double* m64, const size_t* moff) {
+int PrepareGateBlock(unsigned n, const unsigned* tile, unsigned nt,
+                     unsigned nops, const unsigned* ks, const unsigned* qs,
+                     const double* m64, const size_t* moff, BlockGeom& g,
+                     BlockProgram& p, std::vector<uint16_t>& fb) {
   if (nt < kBlockMinBits || nt > kBlockMaxBits || nt > n) {
     SetError("a blocked pass needs %u..%u tile qubits (at most n = %u), got %u",
              kBlockMinBits, kBlockMaxBits, n, nt);
@@ -177,7 +178,7 @@ int LaunchGateBlock(float2* s, unsigned n, const unsigned* tile, unsigned nt,
       return QSIM_HIP_ERR_ARG;
     }
   }
-  BlockGeom g{};
+  g = BlockGeom{};
   g.Lc = 0;
   while (g.Lc < nt && tile[g.Lc] == g.Lc) ++g.Lc;
   g.tb = nt;
@@ -202,9 +203,9 @@ int LaunchGateBlock(float2* s, unsigned n, const unsigned* tile, unsigned nt,
     return -1;
   };
 
-  BlockProgram p{};
+  p = BlockProgram{};
   p.nops = nops;
-  std::vector<uint16_t> fb(std::max<size_t>(nops, 1) * kOpFrag);
+  fb.assign(std::max<size_t>(nops, 1) * kOpFrag, 0);
   std::vector<double> u16(2 * 16 * 16);
   for (unsigned i = 0; i < nops; ++i) {
     const unsigned k = ks[i];
@@ -241,21 +242,87 @@ int LaunchGateBlock(float2* s, unsigned n, const unsigned* tile, unsigned nt,
     BuildFragBf16(4, u16.data(), fb.data() + i * kOpFrag);
   }
 
-  const size_t fbytes = fb.size() * sizeof(uint16_t);
-  void* dfrag = MatrixBuffer(fbytes);
-  if (dfrag == nullptr) return QSIM_HIP_ERR_OOM;
-  QHIP_TRY(StageMatrix(dfrag, fb.data(), fbytes));
+  return QSIM_HIP_OK;
+}
 
-  const size_t lds = (size_t{1} << nt) * sizeof(float2);
+int RunGateBlock(float2* s, const BlockGeom& g, const BlockProgram& p,
+                 const uint16_t* dfrag) {
+  const size_t lds = (size_t{1} << g.tb) * sizeof(float2);
   // Two workgroups per CU fit in LDS; a few tiles each.
   const uint64_t blocks =
       std::min<uint64_t>(g.ntiles, uint64_t(NumCUs()) * 2 * 8);
   PassTimerBegin();
   hipLaunchKernelGGL(GateBlockKernel, dim3(unsigned(blocks)), dim3(kThreads),
-                     lds, Stream(), s, g, p,
-                     static_cast<const uint16_t*>(dfrag));
+                     lds, Stream(), s, g, p, dfrag);
   PassTimerEnd();
   QHIP_LAUNCHED("GateBlockKernel");
+  return QSIM_HIP_OK;
+}
+
+int LaunchGateBlock(float2* s, unsigned n, const unsigned* tile, unsigned nt,
+                    unsigned nops, const unsigned* ks, const unsigned* qs,
+                    const double* m64, const size_t* moff) {
+  BlockGeom g{};
+  BlockProgram p{};
+  std::vector<uint16_t> fb;
+  QHIP_TRY(PrepareGateBlock(n, tile, nt, nops, ks, qs, m64, moff, g, p, fb));
+  const size_t fbytes = fb.size() * sizeof(uint16_t);
+  void* dfrag = MatrixBuffer(fbytes);
+  if (dfrag == nullptr) return QSIM_HIP_ERR_OOM;
+  QHIP_TRY(StageMatrix(dfrag, fb.data(), fbytes));
+  return RunGateBlock(s, g, p, static_cast<const uint16_t*>(dfrag));
+}
+
+// A blocked pass prepared once: geometry, program and device fragments.
+struct PreparedBlock {
+  unsigned n;
+  BlockGeom g;
+  BlockProgram p;
+  uint16_t* dfrag;
+};
+
+int PrepareBlockHandle(unsigned n, const unsigned* tile, unsigned nt,
+                       unsigned nops, const unsigned* ks, const unsigned* qs,
+                       const double* m64, const size_t* moff, void** out) {
+  auto* b = new PreparedBlock{};
+  std::vector<uint16_t> fb;
+  int rc = PrepareGateBlock(n, tile, nt, nops, ks, qs, m64, moff, b->g, b->p,
+                            fb);
+  if (rc != QSIM_HIP_OK) {
+    delete b;
+    return rc;
+  }
+  b->n = n;
+  const size_t bytes = fb.size() * sizeof(uint16_t);
+  hipError_t e = hipMalloc(&b->dfrag, bytes);
+  if (e == hipSuccess) {
+    e = hipMemcpy(b->dfrag, fb.data(), bytes, hipMemcpyHostToDevice);
+  }
+  if (e != hipSuccess) {
+    (void)hipFree(b->dfrag);
+    delete b;
+    return HipFail(e, "prepared blocked pass");
+  }
+  *out = b;
+  return QSIM_HIP_OK;
+}
+
+int RunBlockHandle(void* h, float2* s, unsigned n) {
+  auto* b = static_cast<PreparedBlock*>(h);
+  if (b->n != n) {
+    SetError("prepared blocked pass is for %u qubits, state has %u", b->n, n);
+    return QSIM_HIP_ERR_ARG;
+  }
+  return RunGateBlock(s, b->g, b->p, b->dfrag);
+}
+
+int FreeBlockHandle(void* h) {
+  auto* b = static_cast<PreparedBlock*>(h);
+  if (b == nullptr) return QSIM_HIP_OK;
+  // Launches that read the fragments may still be in flight.
+  QHIP_CHECK(hipStreamSynchronize(Stream()));
+  QHIP_CHECK(hipFree(b->dfrag));
+  delete b;
   return QSIM_HIP_OK;
 }
 

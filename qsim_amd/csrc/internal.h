@@ -192,4 +192,11 @@ int LaunchGateBlock(float2* s, unsigned n, const unsigned* tile, unsigned nt,
                     unsigned nops, const unsigned* ks, const unsigned* qs,
                     const double* m64, const size_t* moff);
 
+// The same pass prepared once (device fragments kept) and run many times.
+int PrepareBlockHandle(unsigned n, const unsigned* tile, unsigned nt,
+                       unsigned nops, const unsigned* ks, const unsigned* qs,
+                       const double* m64, const size_t* moff, void** out);
+int RunBlockHandle(void* h, float2* s, unsigned n);
+int FreeBlockHandle(void* h);
+
 }  // namespace qhip

@@ -20,6 +20,8 @@ DRIVER_PATH = os.path.join(HERE, "lib", "libqsim_hip_driver.so")
 FUSER_MQUBIT = 0
 FUSER_LEARNED = 1
 
+NUM_FEATURES = 18  # include/qsim_amd/learned_planner.h
+
 PASS_GATE = 0
 PASS_MEASUREMENT = 1
 PASS_OPERATION = 2
@@ -51,6 +53,11 @@ def driver_lib():
                 ("qsim_driver_pass_info", i, [vp, u, up, up, up]),
                 ("qsim_driver_block_info", i, [vp, u, up, up, up,
                                                ctypes.POINTER(ctypes.c_double)]),
+                ("qsim_driver_set_method", i, [vp, i]),
+                ("qsim_driver_pass_features", i, [vp, u, i, vp]),
+                ("qsim_driver_plan_learned", i, [vp, vp, u, vp, vp, vp, vp,
+                                                 vp, vp, up, vp]),
+                ("qsim_driver_set_cost_model", None, [vp]),
                 ("qsim_driver_run_passes", i, [vp, vp, i, u, u, u]),
                 ("qsim_driver_run", i, [vp, vp, i, u])):
             f = getattr(d, name)
@@ -80,6 +87,50 @@ class Circuit:
         """Cache-blocked passes (FP64 fused matrices, BF16x3 MFMA)."""
         _check(driver_lib().qsim_driver_fuse_blocked(self.h, tile_bits,
                                                      low_bits))
+        return self.num_passes
+
+    def set_method(self, method: int):
+        """Gate method the partition runs with (-1: the library's)."""
+        _check(driver_lib().qsim_driver_set_method(self.h, method))
+
+    def pass_features(self, method: int) -> np.ndarray:
+        """Cost-model features of every pass, (num_passes, 18) float32."""
+        out = np.zeros((self.num_passes, NUM_FEATURES), np.float32)
+        for i in range(self.num_passes):
+            _check(driver_lib().qsim_driver_pass_features(
+                self.h, i, method, out[i].ctypes.data))
+        return out
+
+    def plan_learned(self, model=None):
+        """Runs the learned planner (model: a qsim_amd.costmodel.CostModel
+        or None for the fallback); the circuit keeps the chosen partition.
+        Returns (chosen index, [candidate dicts])."""
+        m = 32
+        blocked = np.zeros(m, np.int32)
+        a = np.zeros(m, np.uint32)
+        b = np.zeros(m, np.uint32)
+        meth = np.zeros(m, np.int32)
+        pred = np.zeros(m, np.float64)
+        npass = np.zeros(m, np.uint32)
+        count = ctypes.c_uint32(0)
+        chosen = ctypes.c_int(0)
+        _check(driver_lib().qsim_driver_plan_learned(
+            self.h, model.h if model is not None else None, m,
+            blocked.ctypes.data, a.ctypes.data, b.ctypes.data,
+            meth.ctypes.data, pred.ctypes.data, npass.ctypes.data,
+            ctypes.byref(count), ctypes.byref(chosen)))
+        cands = [{"blocked": bool(blocked[j]), "a": int(a[j]), "b": int(b[j]),
+                  "method": int(meth[j]), "predicted_ms": float(pred[j]),
+                  "passes": int(npass[j])} for j in range(count.value)]
+        return chosen.value, cands
+
+    def apply_candidate(self, cand):
+        """Partitions the circuit as planner candidate `cand` describes."""
+        if cand["blocked"]:
+            self.fuse_blocked(cand["a"], cand["b"])
+        else:
+            self.fuse(cand["a"], FUSER_MQUBIT, f64_matrices=True)
+        self.set_method(cand["method"])
         return self.num_passes
 
     def pass_kind(self, i: int) -> int:

@@ -142,3 +142,24 @@ def test_block_rejects_complex128(sim):
     with pytest.raises(qsim_amd.QsimHipError) as e:
         sim.apply_gate_block(list(range(10)), gates, st)
     assert e.value.code == -4
+
+
+def test_prepared_block_runs_repeatedly(sim):
+    # A prepared pass (fragments uploaded once) gives the one-shot pass's
+    # result bit for bit, run after run, and refuses a state of another size.
+    rng = np.random.default_rng(12)
+    n = 18
+    tile = list(range(5)) + [6, 9, 10, 13, 15, 16, 17, 12]
+    gates = random_program(sorted(tile), 20, rng)
+    s = random_state(n, rng)
+    a, b = upload(s, n), upload(s, n)
+    pb = sim.prepare_gate_block(n, tile, gates)
+    for _ in range(3):
+        pb.run(a)
+        sim.apply_gate_block(tile, gates, b)
+        assert np.array_equal(a.to_numpy(), b.to_numpy())
+    other = upload(random_state(n - 1, rng), n - 1)
+    with pytest.raises(qsim_amd.QsimHipError) as e:
+        pb.run(other)
+    assert e.value.code == -1
+    pb.free()

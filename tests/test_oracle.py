@@ -236,3 +236,69 @@ def test_parser_matches_gate_calls():
     q, m = oracle.gate_matrix("rx", [1], [0.7], np.float64)
     oracle.apply_controlled_gate(3, ref, q, [0, 2], 3, m)
     np.testing.assert_allclose(s, ref, atol=1e-15)
+
+
+# ---- cost-model network (mlp_oracle.cc) --------------------------------------
+
+def _mlp_layers(F, H):
+    return [(F, H), (H, H), (H, H), (H, 1)]
+
+
+def _mlp_pack(layers):
+    return np.concatenate([np.concatenate([w.reshape(-1), b]) for w, b in
+                           layers]).astype(np.float32)
+
+
+def test_mlp_forward_closed_form():
+    # Identity-like hidden layers pass the positive part of the input through;
+    # the output sums it: y = sum(relu(x + b1)) + b_out.
+    F, H = 3, 4
+    w1 = np.zeros((F, H))
+    w1[np.arange(F), np.arange(F)] = 1.0
+    b1 = np.array([0.0, 0.0, -1.0, 0.25])
+    eye = np.eye(H)
+    p = _mlp_pack([(w1, b1), (eye, np.zeros(H)), (eye, np.zeros(H)),
+                   (np.ones((H, 1)), np.array([0.5]))])
+    assert len(p) == oracle.mlp_num_params(F, H)
+    x = np.array([[1.0, -2.0, 3.0], [-1.0, 0.5, 0.5], [0.0, 0.0, 0.0]])
+    want = [1 + 0 + 2 + 0.25 + 0.5, 0 + 0.5 + 0 + 0.25 + 0.5, 0.25 + 0.5]
+    assert np.allclose(oracle.mlp_forward(F, H, p, x), want, atol=1e-12)
+
+
+def test_mlp_gradient_matches_finite_differences():
+    # Central differences of the FP64 loss, with parameters on a 2^-20 grid
+    # and a 2^-14 step so that p +- h is exact in float32; every component of
+    # the analytic gradient must agree.
+    rng = np.random.default_rng(17)
+    F, H, B = 5, 6, 7
+    n = oracle.mlp_num_params(F, H)
+    p = np.round(rng.normal(scale=0.6, size=n) * 2**20) / 2**20
+    p = p.astype(np.float32)
+    x = rng.normal(size=(B, F)).astype(np.float32)
+    t = rng.normal(size=B).astype(np.float32)
+    loss, g = oracle.mlp_gradient(F, H, p, x, t)
+    h = 2.0 ** -14
+    fd = np.zeros(n)
+    for i in range(n):
+        pp, pm = p.copy(), p.copy()
+        pp[i] += np.float32(h)
+        pm[i] -= np.float32(h)
+        fd[i] = (oracle.mlp_gradient(F, H, pp, x, t)[0] -
+                 oracle.mlp_gradient(F, H, pm, x, t)[0]) / (2 * h)
+    err = np.abs(fd - g)
+    assert np.all(err <= 1e-6 * np.abs(g) + 1e-7), (err.max(), np.argmax(err))
+    assert abs(loss - np.mean((oracle.mlp_forward(F, H, p, x) - t) ** 2)) < 1e-12
+
+
+def test_mlp_gradient_check_catches_a_transposed_weight():
+    # A planted mistake (one layer's weight read transposed) is caught by the
+    # finite-difference check above: the losses differ.
+    rng = np.random.default_rng(3)
+    F, H = 4, 4
+    layers = [(rng.normal(size=(i, o)), rng.normal(size=o))
+              for i, o in _mlp_layers(F, H)]
+    p = _mlp_pack(layers)
+    bad = _mlp_pack([layers[0], (layers[1][0].T, layers[1][1])] + layers[2:])
+    x = rng.normal(size=(5, F)).astype(np.float32)
+    assert not np.allclose(oracle.mlp_forward(F, H, p, x),
+                           oracle.mlp_forward(F, H, bad, x))
