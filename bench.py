@@ -13,6 +13,10 @@ apply loop over the gfx950 kernels.  The paths run on the same circuit:
   blocked-13         cache-blocked passes over 13 tile qubits (5 lowest + 8
                      chosen), gates fused to <= 4 qubits in FP64 inside each
                      pass, split-BF16 MFMA on the LDS-resident tile
+  learned            the partition the learned planner picks for this
+                     circuit with the committed cost model
+                     (qsim_amd/data/cost_model.json), among qsim's fuser at
+                     max_fused_size 2..6 x gate methods and blocked passes
 
 Each path's fidelity F = |<psi64|psi>|^2 / (<psi64|psi64> <psi|psi>) is
 measured on the same construction at 28 qubits against a complex128 run of
@@ -55,6 +59,8 @@ PATHS = {
     "fp32-mfma": ("mqubit", 4, False, 0),
     "bf16-k6": ("mqubit", 6, True, 2),
     "blocked-13": ("blocked", 13, 5, 0),
+    # the learned planner's pick (qsim_amd/data/cost_model.json)
+    "learned": ("learned", 0, 0, 0),
 }
 
 # One 32-group batch of a 4-qubit op in a blocked pass: 8 MFMAs of
@@ -67,12 +73,29 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+_MODEL = None
+
+
 def fuse(circ, name):
-    from qsim_amd import driver
-    part, a, b, _ = PATHS[name]
+    """Partitions the circuit for path `name`; returns (passes, gate method,
+    planner pick or None)."""
+    global _MODEL
+    from qsim_amd import costmodel, driver
+    part, a, b, method = PATHS[name]
+    if part == "learned":
+        if _MODEL is None:
+            _MODEL = costmodel.CostModel.load()
+        chosen, cands = circ.plan_learned(_MODEL)
+        return circ.num_passes, cands[chosen]["method"], cands[chosen]
     if part == "blocked":
-        return circ.fuse_blocked(a, b)
-    return circ.fuse(a, driver.FUSER_MQUBIT, f64_matrices=b)
+        return circ.fuse_blocked(a, b), method, None
+    return circ.fuse(a, driver.FUSER_MQUBIT, f64_matrices=b), method, None
+
+
+def available_paths():
+    from qsim_amd import costmodel
+    return [p for p in PATHS
+            if PATHS[p][0] != "learned" or os.path.exists(costmodel.MODEL_PATH)]
 
 
 def kernel_family(circ, i, qubits, method, n):
@@ -94,11 +117,16 @@ def run_path(args, text, name):
     import qsim_amd
     from qsim_amd import driver
 
-    part, kmax, f64m, method = PATHS[name]
+    part, kmax, f64m, _ = PATHS[name]
     circ = driver.Circuit(text)
     t0 = time.time()
-    npass = fuse(circ, name)
+    npass, method, pick = fuse(circ, name)
     fuse_s = time.time() - t0
+    if pick is not None:
+        log(f"[{name}] planner pick: {pick}")
+        part = "blocked" if pick["blocked"] else "mqubit"
+        kmax = pick["a"]
+        f64m = pick["b"] if pick["blocked"] else True
     passes = circ.passes()
     n = circ.num_qubits
     block_ops = [len(circ.block_info(i)) if
@@ -167,7 +195,7 @@ def run_path(args, text, name):
                         "gbps": bytes_per_pass / (float(np.mean(v)) * 1e-3) / 1e9}
                for k, v in sorted(per_k.items())}
     return {
-        "name": name, "partition": part,
+        "name": name, "partition": part, "planner_pick": pick,
         "max_fused_size": kmax if part == "mqubit" else 4,
         "fp64_matrices": bool(f64m) if part == "mqubit" else True,
         "tile_qubits": kmax if part == "blocked" else None,
@@ -207,9 +235,8 @@ def fidelities(args, names):
     ss32 = qsim_amd.StateSpace(np.float32)
     tmp = ss64.create(nq)
     for name in names:
-        method = PATHS[name][3]
         c = driver.Circuit(text)
-        fuse(c, name)
+        _, method, _ = fuse(c, name)
         qsim_amd.set_gate_method(method)
         st = ss32.create(nq)
         try:
@@ -268,7 +295,9 @@ def main():
     ap.add_argument("--qubits", type=int, default=32)
     ap.add_argument("--cycles", type=int, default=20)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--paths", default=",".join(PATHS))
+    ap.add_argument("--paths", default="",
+                    help="comma-separated subset of " + ",".join(PATHS) +
+                    " (default: all; 'learned' needs the trained model)")
     ap.add_argument("--fidelity-qubits", type=int, default=28)
     ap.add_argument("--no-fidelity", action="store_true")
     ap.add_argument("--cpu-passes", type=int, default=6)
@@ -288,7 +317,7 @@ def main():
     log(f"device: {name}, {mem / 2**30:.0f} GiB, {cus} CUs")
     text = circuits.grid_circuit(args.qubits, args.cycles, args.seed)
     rows, cols = circuits.grid_shape(args.qubits)
-    names = [p for p in args.paths.split(",") if p]
+    names = [p for p in args.paths.split(",") if p] or available_paths()
     results = {p: run_path(args, text, p) for p in names}
 
     fid = {}
@@ -335,8 +364,10 @@ def main():
                          f"{args.cycles} cycles, seed {args.seed}, complex64, "
                          f"path {best}"),
             "num_qubits": args.qubits, "cycles": args.cycles,
-            "method": best, "partition": PATHS[best][0],
+            "method": best, "partition": b["partition"],
+            "planner_pick": b["planner_pick"],
             "max_fused_size": b["max_fused_size"],
+            "tile_qubits": b["tile_qubits"],
             "num_gates": b["num_gates"], "fused_passes": b["passes"],
         },
         "gate_passes_per_s": b["gate_passes_per_s"],

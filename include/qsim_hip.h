@@ -258,6 +258,63 @@ int qsim_hip_event_elapsed_ms(int slot_start, int slot_end, float* ms);
 int qsim_hip_pass_timer_start(unsigned capacity);
 int qsim_hip_pass_timer_stop(float* ms, unsigned max_count, unsigned* count);
 
+/* ---- batched noisy trajectories -------------------------------------------
+ * Many quantum trajectories (of possibly different qubit counts) run in ONE
+ * kernel launch, one workgroup per trajectory, each with its own state in a
+ * device pool and its own operation stream: fused gates (FP32 matrices, the
+ * Kraus operators already sampled on the host folded in) and amplitude-
+ * damping decisions that need the state's norm (taken on the device, in
+ * FP64).  At the end every trajectory writes <Z_a Z_b> for its qubit pairs
+ * (FP64).  Replaces the serial loop of QuantumTrajectorySimulator::RunBatch
+ * / RunIteration (lib/qtrajectory.h:112-387), with the same sampling rule:
+ * a channel's Kraus operator is chosen from the draw against probability
+ * bounds, and against the norms of the updated state when the bounds do not
+ * decide (lib/qtrajectory.h:296-356). */
+enum { QSIM_HIP_TRAJ_GATE = 0, QSIM_HIP_TRAJ_DAMP_DECIDE = 1 };
+
+typedef struct {
+  uint32_t kind;   /* QSIM_HIP_TRAJ_* */
+  uint32_t k;      /* gate: qubit count (1..4) */
+  uint32_t q[4];   /* gate: ascending qubits; decision: q[0] */
+  uint32_t moff;   /* gate: matrix offset (floats); decision: K0 (2x2) */
+  uint32_t moff2;  /* decision: K1 (2x2) */
+  float r;         /* decision: the trajectory's draw for the channel */
+  float kdk1;      /* decision: (K0^dagger K0)[1][1] */
+  uint32_t slot;   /* decision: index in the decision record */
+  uint32_t pad;    /* decision: bit 0 = normalise the state first (the
+                      trajectory has applied a non-unitary operator) */
+} qsim_hip_traj_op;
+
+typedef struct {
+  uint32_t num_qubits;
+  uint32_t num_pairs;     /* <Z_a Z_b> pairs at pairs[2 * pair_offset] */
+  uint64_t state_offset;  /* complex amplitudes from the pool start */
+  uint64_t op_begin;      /* operations [op_begin, op_end) */
+  uint64_t op_end;
+  uint32_t pair_offset;
+  uint32_t pad;           /* bit 0: normalise the final state */
+} qsim_hip_traj_desc;
+
+/* Copies a batch to the device and allocates its state pool (states of all
+ * trajectories at once).  Trajectories run in the order given: put the
+ * largest first.  num_decisions: size of the decision record. */
+int qsim_hip_traj_prepare(const qsim_hip_traj_desc* trajs, unsigned num_traj,
+                          const qsim_hip_traj_op* ops, uint64_t num_ops,
+                          const float* matrices, uint64_t num_floats,
+                          const uint32_t* pairs, uint32_t num_pairs,
+                          uint32_t num_decisions, void** handle);
+/* One launch of the whole batch with `threads` (256, 512 or 1024) per
+ * trajectory; with use_graph the launch is captured once in a hipGraph and
+ * replayed after.  Waits for completion. */
+int qsim_hip_traj_run(void* handle, unsigned threads, int use_graph);
+/* <Z_a Z_b> of every pair (FP64, pair order of the batch) and the decision
+ * record (0: K0, 1: K1, -1: not reached). */
+int qsim_hip_traj_results(void* handle, double* zz, uint32_t num_pairs,
+                          int32_t* decisions, uint32_t num_decisions);
+/* Final state of trajectory t (complex64, 2^n pairs) to host memory. */
+int qsim_hip_traj_state(void* handle, unsigned t, float* out);
+int qsim_hip_traj_free(void* handle);
+
 /* ---- pass-time cost model ------------------------------------------------
  * A multilayer perceptron num_features -> hidden -> hidden -> hidden -> 1
  * (ReLU hidden layers) that predicts a pass's log time from features of the

@@ -100,6 +100,35 @@ int qsim_driver_plan_learned(void* circuit, void* model,
  * (NULL: the fallback partition). */
 void qsim_driver_set_cost_model(void* model);
 
+/* ---- batched noisy trajectories (qsim_amd/apps/trajectories.cc) ---------
+ * Noisy versions of qsim circuits on rows x cols grids: a depolarising
+ * channel (p_depol) on each qubit of every gate after it, an amplitude-
+ * damping channel (gamma_damp) on every qubit after each cycle (gate times
+ * 2c, 2c + 1).  Each trajectory replays RunIteration's sampling
+ * (lib/qtrajectory.h:240-387) from mt19937(rep) on the host where the
+ * probability bounds decide it; the rest is decided on the device.  All
+ * trajectories then run in one launch (qsim_hip_traj_run).  Replaces
+ * QuantumTrajectorySimulator::RunBatch (lib/qtrajectory.h:112-175) over
+ * reps; results are <Z_a Z_b> on grid neighbours (horizontal pairs then
+ * vertical ones, row by row) and the Kraus index of every channel
+ * (collect_kop_stat order without the gates' zeros). */
+void* qsim_traj_create(const char* const* circuits, unsigned num_circuits,
+                       const unsigned* rows, const unsigned* cols,
+                       double p_depol, double gamma_damp);
+unsigned qsim_traj_num_channels(void* batch, unsigned circuit);
+/* Trajectory t runs circuit circuit_of[t] with repetition id reps[t]; runs
+ * of deferred gates are fused to at most max_fused_size (1..4) qubits. */
+int qsim_traj_plan(void* batch, const unsigned* circuit_of,
+                   const uint64_t* reps, unsigned num_traj,
+                   unsigned max_fused_size);
+int qsim_traj_run(void* batch, unsigned threads, int use_graph);
+int qsim_traj_zz(void* batch, unsigned t, double* zz, unsigned max,
+                 unsigned* num);
+int qsim_traj_kraus(void* batch, unsigned t, int32_t* kidx, unsigned max,
+                    unsigned* num);
+int qsim_traj_state(void* batch, unsigned t, float* out);
+void qsim_traj_free(void* batch);
+
 /* Applies passes [first, first + count) to a state of the circuit's size
  * (the apply loop of QSimRunner::Run, lib/run_qsim.h:264-280); measurement
  * passes draw from mt19937(seed).  Asynchronous for gate passes. */

@@ -289,3 +289,21 @@ def reference_draws(num, seed, max_value, path):
     subprocess.run([REF_BIN, "-R", str(num), "-x", str(seed), "-M",
                     repr(float(max_value)), "-b", path], check=True)
     return np.fromfile(path, dtype=np.float64)
+
+
+def run_reference_trajectories(circuit_file, rows, cols, rep0, rep1,
+                               p_depol=1e-3, gamma=5e-3, max_fused=4,
+                               threads=1, timeout=3600):
+    """qsim's QuantumTrajectorySimulator on the noisy grid circuit
+    (oracle/_ref/qsim_ref -T): per repetition the Kraus index of every
+    channel and <Z_a Z_b> of the grid-neighbour pairs; plus a summary."""
+    import json
+    if not os.path.exists(REF_BIN):
+        raise FileNotFoundError(REF_BIN)
+    cmd = [REF_BIN, "-c", circuit_file, "-T", f"{rep0}:{rep1}", "-g",
+           f"{rows}:{cols}", "-p", repr(float(p_depol)), "-G",
+           repr(float(gamma)), "-f", str(max_fused), "-t", str(threads)]
+    out = subprocess.run(cmd, check=True, capture_output=True, text=True,
+                         timeout=timeout)
+    lines = [json.loads(s) for s in out.stdout.strip().splitlines()]
+    return lines[:-1], lines[-1]

@@ -9,6 +9,8 @@
 //            [-s SEED] [-o STATE.bin] [-n NUM_SAMPLES] [-x SAMPLE_SEED]
 //            [-b BITSTRINGS.bin] [-d MAXTIME] [-P PASSES]
 //   qsim_ref -R NUM -x SEED -M MAX -b DRAWS.bin     (GenerateRandomValues)
+//   qsim_ref -c FILE -T R0:R1 -g ROWS:COLS [-p P] [-G GAMMA] [-f MAX_FUSED]
+//            [-t THREADS]       (noisy trajectories, QuantumTrajectorySimulator)
 //
 // The simulation is QSimRunner::Run with MultiQubitGateFuser
 // (lib/run_qsim.h:225-289, lib/fuser_mqubit.h:185-526) on the state of the
@@ -35,6 +37,8 @@
 #include "simmux.h"
 #include "simulator_basic.h"
 #include "util.h"
+#include "channels_cirq.h"
+#include "qtrajectory.h"
 
 namespace {
 
@@ -52,6 +56,11 @@ struct Options {
   uint64_t draws = 0;
   double draws_max = 1.0;
   long passes = -1;
+  // Trajectory mode (-T R0:R1 -g ROWS:COLS -p P_DEPOL -G GAMMA).
+  bool trajectories = false;
+  uint64_t rep0 = 0, rep1 = 0;
+  unsigned rows = 0, cols = 0;
+  double p_depol = 1e-3, gamma = 5e-3;
 };
 
 template <typename Sim, typename FP>
@@ -169,13 +178,121 @@ int Run(const Options& opt) {
   return 0;
 }
 
+// Noisy grid circuit, by the rule the trajectory engine states
+// (qsim_amd/apps/trajectories.cc): gate times t -> 3t; after every gate a
+// depolarising channel on each of its qubits (time 3t + 1); after the last
+// gate of cycle c (gate times 2c, 2c + 1) an amplitude-damping channel on
+// every qubit (time 3(2c + 1) + 2).
+qsim::Circuit<qsim::Operation<float>> NoisyGrid(
+    const qsim::Circuit<qsim::Operation<float>>& c, double p, double gamma) {
+  using namespace qsim;
+  Circuit<Operation<float>> nc;
+  nc.num_qubits = c.num_qubits;
+  for (size_t i = 0; i < c.ops.size(); ++i) {
+    Gate<float> g = *OpGetAlternative<Gate<float>>(c.ops[i]);
+    const unsigned t = g.time;
+    g.time = 3 * t;
+    const std::vector<unsigned> qs = g.qubits;
+    nc.ops.push_back(std::move(g));
+    for (unsigned q : qs) {
+      nc.ops.push_back(Cirq::DepolarizingChannel<float>::Create(3 * t + 1, q, p));
+    }
+    if (i + 1 == c.ops.size() || OpTime(c.ops[i + 1]) / 2 != t / 2) {
+      for (unsigned q = 0; q < c.num_qubits; ++q) {
+        nc.ops.push_back(Cirq::AmplitudeDampingChannel<float>::Create(
+            3 * (2 * (t / 2) + 1) + 2, q, gamma));
+      }
+    }
+  }
+  return nc;
+}
+
+// qsim's QuantumTrajectorySimulator (lib/qtrajectory.h) on SimulatorAVX for
+// repetitions [r0, r1): one JSON line per trajectory with the Kraus index of
+// every channel (collect_kop_stat, gates' zeros dropped) and <Z_a Z_b> of the
+// grid-neighbour pairs (horizontal then vertical per cell, row by row).
+int RunTrajectories(const Options& opt) {
+  using namespace qsim;
+  using Sim = qsim::Simulator<ParallelFor>;
+  using Fac = Factory<Sim, float>;
+  using Runner = QSimRunner<IO, MultiQubitGateFuser<IO>, Fac>;
+  using QT = QuantumTrajectorySimulator<IO, Runner>;
+  Circuit<Operation<float>> circuit;
+  if (!CircuitQsimParser<IOFile>::FromFile(opt.maxtime, opt.circuit_file,
+                                           circuit)) {
+    return 1;
+  }
+  const unsigned rows = opt.rows, cols = opt.cols;
+  if (rows * cols != circuit.num_qubits) return 1;
+  auto nc = NoisyGrid(circuit, opt.p_depol, opt.gamma);
+  std::vector<char> is_channel;
+  for (const auto& op : nc.ops) {
+    is_channel.push_back(OpGetAlternative<Channel<float>>(op) != nullptr);
+  }
+  std::vector<std::pair<unsigned, unsigned>> pairs;
+  for (unsigned r = 0; r < rows; ++r) {
+    for (unsigned c = 0; c < cols; ++c) {
+      if (c + 1 < cols) pairs.push_back({r * cols + c, r * cols + c + 1});
+      if (r + 1 < rows) pairs.push_back({r * cols + c, (r + 1) * cols + c});
+    }
+  }
+  const std::vector<float> zz = {1, 0, 0, 0, 0, 0, 0, 0, 0, 0, -1, 0, 0, 0, 0, 0,
+                                 0, 0, 0, 0, -1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0};
+  Fac factory(opt.threads);
+  auto ss = factory.CreateStateSpace();
+  auto sim = factory.CreateSimulator();
+  typename QT::Parameter param;
+  param.max_fused_size = opt.max_fused;
+  param.verbosity = 0;
+  param.collect_kop_stat = true;
+  double t_sim = 0;
+  auto measure = [&](uint64_t rep, const typename Fac::StateSpace::State& st,
+                     const typename QT::Stat& stat) {
+    std::printf("{\"rep\": %llu, \"kraus\": [", (unsigned long long)rep);
+    bool first = true;
+    for (size_t i = 0; i < stat.samples.size() && i < is_channel.size(); ++i) {
+      if (!is_channel[i]) continue;
+      std::printf("%s%llu", first ? "" : ", ",
+                  (unsigned long long)stat.samples[i]);
+      first = false;
+    }
+    std::printf("], \"zz\": [");
+    for (size_t j = 0; j < pairs.size(); ++j) {
+      auto v = sim.ExpectationValue({pairs[j].first, pairs[j].second},
+                                    zz.data(), st);
+      std::printf("%s%.17g", j ? ", " : "", std::real(v));
+    }
+    std::printf("]}\n");
+  };
+  double t0 = GetTime();
+  bool ok = QT::RunBatch(param, nc, opt.rep0, opt.rep1, ss, sim, measure);
+  t_sim = GetTime() - t0;
+  std::printf("{\"trajectories\": %llu, \"num_qubits\": %u, \"sim_s\": %.9g, "
+              "\"threads\": %u}\n",
+              (unsigned long long)(opt.rep1 - opt.rep0), circuit.num_qubits,
+              t_sim, opt.threads);
+  return ok ? 0 : 1;
+}
+
 }  // namespace
 
 int main(int argc, char* argv[]) {
   Options opt;
   int k;
-  while ((k = getopt(argc, argv, "c:m:f:t:s:o:n:x:b:d:R:M:P:")) != -1) {
+  while ((k = getopt(argc, argv, "c:m:f:t:s:o:n:x:b:d:R:M:P:T:g:p:G:")) != -1) {
     switch (k) {
+      case 'T':
+        opt.trajectories = true;
+        if (std::sscanf(optarg, "%llu:%llu",
+                        reinterpret_cast<unsigned long long*>(&opt.rep0),
+                        reinterpret_cast<unsigned long long*>(&opt.rep1)) != 2)
+          return 1;
+        break;
+      case 'g':
+        if (std::sscanf(optarg, "%u:%u", &opt.rows, &opt.cols) != 2) return 1;
+        break;
+      case 'p': opt.p_depol = std::strtod(optarg, nullptr); break;
+      case 'G': opt.gamma = std::strtod(optarg, nullptr); break;
       case 'c': opt.circuit_file = optarg; break;
       case 'm': opt.mode = optarg; break;
       case 'f': opt.max_fused = std::atoi(optarg); break;
@@ -208,6 +325,7 @@ int main(int argc, char* argv[]) {
   }
 
   using For = qsim::ParallelFor;
+  if (opt.trajectories) return RunTrajectories(opt);
   if (opt.mode == "avx") return Run<qsim::Simulator<For>, float>(opt);
   if (opt.mode == "basic32") return Run<qsim::SimulatorBasic<For, float>, float>(opt);
   if (opt.mode == "basic64") return Run<qsim::SimulatorBasic<For, double>, double>(opt);

@@ -91,6 +91,12 @@ SIGNATURES = [
                                     ctypes.POINTER(_vp)]),
     ("qsim_hip_block_run", _i, [_vp, _i, _vp, _u]),
     ("qsim_hip_block_free", _i, [_vp]),
+    ("qsim_hip_traj_prepare", _i, [_vp, _u, _vp, _u64, _vp, _u64, _u32p, _u,
+                                   _u, ctypes.POINTER(_vp)]),
+    ("qsim_hip_traj_run", _i, [_vp, _u, _i]),
+    ("qsim_hip_traj_results", _i, [_vp, _f64p, _u, _vp, _u]),
+    ("qsim_hip_traj_state", _i, [_vp, _u, _vp]),
+    ("qsim_hip_traj_free", _i, [_vp]),
     ("qsim_hip_costmodel_create", _i, [_u, _u, ctypes.POINTER(_vp)]),
     ("qsim_hip_costmodel_free", _i, [_vp]),
     ("qsim_hip_costmodel_num_params", _u64, [_vp]),

@@ -174,8 +174,13 @@ int RunF32M64(DriverCircuit& d, float* p, unsigned first, unsigned count,
       auto r = ss.Measure(m->qubits, rgen, state);
       if (!r.valid) return QSIM_HIP_ERR_ARG;
     } else {
+      // Unfusible operations: measurements (qsim's fuser hands them back as
+      // pointers to the circuit's operations) and controlled gates.
       const Operation<double>* o = std::get<const Operation<double>*>(op);
-      if (const auto* g = OpGetAlternative<Gate<double>>(*o)) {
+      if (const auto* m = OpGetAlternative<Measurement>(*o)) {
+        auto r = ss.Measure(m->qubits, rgen, state);
+        if (!r.valid) return QSIM_HIP_ERR_ARG;
+      } else if (const auto* g = OpGetAlternative<Gate<double>>(*o)) {
         sim.ApplyGate(g->qubits, g->matrix.data(), state);
       } else if (const auto* cg = OpGetAlternative<ControlledGate<double>>(*o)) {
         int rc = qsim_hip_apply_controlled_gate(

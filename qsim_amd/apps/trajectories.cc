@@ -1,0 +1,405 @@
+// trajectories.cc -- host side of the batched trajectory engine
+// (include/qsim_hip_driver.h, qsim_traj_*).
+//
+// For every trajectory this replays qsim's RunIteration
+// (lib/qtrajectory.h:240-387) on the host as far as the host can: the same
+// mt19937(rep) stream with one uniform draw per operation, the same walk over
+// each channel's Kraus operators against their probability bounds, the same
+// deferral of the chosen operators.  Runs of deferred gates are fused with
+// qsim's MultiQubitGateFuser (FP64 products) into the trajectory's operation
+// stream.  Where the bounds do not decide (an amplitude-damping draw past
+// 1 - gamma) the stream gets a decision operation that the device takes from
+// the state itself.  All streams then run in one launch
+// (qsim_hip_traj_run).
+//
+// The noisy circuit is built from a qsim circuit by one fixed rule: after
+// every gate, a depolarising channel (DepolarizingChannel,
+// lib/channels_cirq.h:127-200) on each of its qubits; after the last gate of
+// every cycle (gate times 2c and 2c + 1), an amplitude-damping channel
+// (AmplitudeDampingChannel, lib/channels_cirq.h:274-300) on every qubit.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <sstream>
+#include <string>
+#include <unordered_map>
+#include <variant>
+#include <vector>
+
+#include "channel.h"
+#include "channels_cirq.h"
+#include "circuit_qsim_parser.h"
+#include "fuser_mqubit.h"
+#include "io.h"
+#include "qsim_hip.h"
+#include "qsim_hip_driver.h"
+
+namespace {
+
+using namespace qsim;
+
+struct NoisyOp {
+  int kind;  // 0 gate (index into the circuit), 1 channel (index into chans)
+  size_t index;
+};
+
+struct NoisyCircuit {
+  unsigned num_qubits = 0;
+  Circuit<Operation<double>> circuit;
+  std::vector<Channel<double>> chans;
+  std::vector<NoisyOp> ops;
+  std::vector<uint32_t> pairs;  // grid-neighbour pairs (a, b)
+  unsigned num_channels = 0;
+};
+
+// One trajectory's plan.
+struct TrajPlan {
+  unsigned circuit = 0;
+  uint64_t rep = 0;
+  std::vector<qsim_hip_traj_op> ops;
+  std::vector<std::vector<float>> mats;  // per gate op, FP32 matrix
+  std::vector<int32_t> kraus;            // per channel: index, or -2 - slot
+  unsigned num_decisions = 0;
+  bool normalize_last = false;           // non-unitary at the end
+};
+
+struct TrajBatchHost {
+  std::vector<NoisyCircuit> circuits;
+  double p = 0, gamma = 0;
+  std::vector<TrajPlan> plans;      // caller order
+  std::vector<unsigned> order;      // device order -> caller index
+  std::vector<uint32_t> pair_off;   // caller order
+  std::vector<uint32_t> dec_off;    // caller order
+  std::vector<float> damp_k;        // K0 then K1 (2x2, FP32)
+  void* handle = nullptr;
+  uint32_t total_pairs = 0, total_decisions = 0;
+  std::vector<double> zz;
+  std::vector<int32_t> dec;
+  bool have_results = false;
+};
+
+bool Build(const std::string& text, unsigned rows, unsigned cols, double p,
+           double gamma, NoisyCircuit& nc) {
+  std::stringstream ss(text);
+  if (!CircuitQsimParser<IO>::FromStream(~0u, "circuit", ss, nc.circuit)) {
+    return false;
+  }
+  const unsigned n = nc.circuit.num_qubits;
+  if (rows * cols != n) return false;
+  nc.num_qubits = n;
+  const auto& ops = nc.circuit.ops;
+  for (size_t i = 0; i < ops.size(); ++i) {
+    const Gate<double>* g = OpGetAlternative<Gate<double>>(ops[i]);
+    if (g == nullptr || g->qubits.empty() || g->qubits.size() > 4) {
+      return false;
+    }
+    const unsigned t = g->time;
+    nc.ops.push_back({0, i});
+    for (unsigned q : g->qubits) {
+      nc.ops.push_back({1, nc.chans.size()});
+      nc.chans.push_back(
+          Cirq::DepolarizingChannel<double>::Create(3 * t + 1, q, p));
+    }
+    const bool cycle_end =
+        i + 1 == ops.size() || OpTime(ops[i + 1]) / 2 != t / 2;
+    if (cycle_end) {
+      const unsigned c = t / 2;
+      for (unsigned q = 0; q < n; ++q) {
+        nc.ops.push_back({1, nc.chans.size()});
+        nc.chans.push_back(Cirq::AmplitudeDampingChannel<double>::Create(
+            3 * (2 * c + 1) + 2, q, gamma));
+      }
+    }
+  }
+  nc.num_channels = nc.chans.size();
+  for (unsigned r = 0; r < rows; ++r) {
+    for (unsigned c = 0; c < cols; ++c) {
+      if (c + 1 < cols) {
+        nc.pairs.push_back(r * cols + c);
+        nc.pairs.push_back(r * cols + c + 1);
+      }
+      if (r + 1 < rows) {
+        nc.pairs.push_back(r * cols + c);
+        nc.pairs.push_back((r + 1) * cols + c);
+      }
+    }
+  }
+  return true;
+}
+
+// A gate of the noisy circuit with its time remapped so that the deferred
+// gates of a trajectory are time-ordered per qubit (gate 3t, depolarising
+// Paulis 3t + 1, damping 3(2c + 1) + 2).
+Gate<double> Timed(const Gate<double>& g, unsigned time) {
+  Gate<double> h = g;
+  h.time = time;
+  return h;
+}
+
+void Flush(std::vector<Operation<double>>& deferred, unsigned n,
+           unsigned max_fused, TrajPlan& plan) {
+  if (deferred.empty()) return;
+  MultiQubitGateFuser<IO>::Parameter fp;
+  fp.max_fused_size = max_fused;
+  fp.verbosity = 0;
+  auto fused = MultiQubitGateFuser<IO>::FuseGates(fp, n, deferred);
+  for (const auto& f : fused) {
+    const auto* fg = std::get_if<FusedGate<double>>(&f);
+    if (fg == nullptr) continue;
+    qsim_hip_traj_op o{};
+    o.kind = QSIM_HIP_TRAJ_GATE;
+    o.k = fg->qubits.size();
+    for (unsigned b = 0; b < o.k && b < 4; ++b) o.q[b] = fg->qubits[b];
+    plan.ops.push_back(o);
+    plan.mats.emplace_back(fg->matrix.begin(), fg->matrix.end());
+  }
+  deferred.clear();
+}
+
+// RunIteration's sampling (lib/qtrajectory.h:270-356) for one trajectory.
+void PlanTrajectory(const NoisyCircuit& nc, uint64_t rep, unsigned max_fused,
+                    TrajPlan& plan) {
+  const unsigned n = nc.num_qubits;
+  std::mt19937 rgen(rep);
+  std::uniform_real_distribution<double> distr(0.0, 1.0);
+  std::vector<Operation<double>> deferred;
+  plan.ops.clear();
+  plan.mats.clear();
+  plan.kraus.clear();
+  plan.num_decisions = 0;
+  bool unitary = true;
+  for (const NoisyOp& op : nc.ops) {
+    const double r = distr(rgen);
+    if (op.kind == 0) {
+      const auto& g = *OpGetAlternative<Gate<double>>(nc.circuit.ops[op.index]);
+      deferred.push_back(Timed(g, 3 * g.time));
+      continue;
+    }
+    const Channel<double>& ch = nc.chans[op.index];
+    double cp = 0;
+    int picked = -1;
+    for (size_t i = 0; i < ch.kops.size(); ++i) {
+      cp += ch.kops[i].prob;
+      if (r < cp) {
+        picked = int(i);
+        break;
+      }
+    }
+    if (picked >= 0) {
+      for (const auto& g : ch.kops[picked].ops) deferred.push_back(g);
+      plan.kraus.push_back(picked);
+      unitary = unitary && ch.kops[picked].unitary;
+      continue;
+    }
+    // The bounds do not decide: the state does (two-operator damping).  A
+    // non-unitary history is normalised first, as NormalizeState does.
+    Flush(deferred, n, max_fused, plan);
+    qsim_hip_traj_op o{};
+    o.kind = QSIM_HIP_TRAJ_DAMP_DECIDE;
+    o.q[0] = ch.qubits[0];
+    o.r = float(r);
+    o.kdk1 = float(ch.kops[0].kd_k[6]);
+    o.slot = plan.num_decisions++;
+    o.pad = unitary ? 0 : 1;
+    plan.ops.push_back(o);
+    plan.mats.emplace_back();  // no matrix of its own
+    plan.kraus.push_back(-2 - int32_t(o.slot));
+    unitary = false;
+  }
+  Flush(deferred, n, max_fused, plan);
+  plan.normalize_last = !unitary;
+}
+
+}  // namespace
+
+extern "C" {
+
+void* qsim_traj_create(const char* const* circuits, unsigned num_circuits,
+                       const unsigned* rows, const unsigned* cols,
+                       double p_depol, double gamma_damp) {
+  if (circuits == nullptr || rows == nullptr || cols == nullptr ||
+      num_circuits == 0 || p_depol < 0 || p_depol > 1 || gamma_damp < 0 ||
+      gamma_damp > 1) {
+    return nullptr;
+  }
+  auto b = std::make_unique<TrajBatchHost>();
+  b->p = p_depol;
+  b->gamma = gamma_damp;
+  b->circuits.resize(num_circuits);
+  for (unsigned i = 0; i < num_circuits; ++i) {
+    if (circuits[i] == nullptr ||
+        !Build(circuits[i], rows[i], cols[i], p_depol, gamma_damp,
+               b->circuits[i])) {
+      return nullptr;
+    }
+  }
+  // The two damping operators as the channel holds them, rounded to FP32.
+  auto ad = Cirq::AmplitudeDampingChannel<double>::Create(0, 0, gamma_damp);
+  for (int k = 0; k < 2; ++k)
+    for (double v : ad.kops[k].ops[0].matrix) b->damp_k.push_back(float(v));
+  return b.release();
+}
+
+unsigned qsim_traj_num_channels(void* batch, unsigned circuit) {
+  auto* b = static_cast<TrajBatchHost*>(batch);
+  return circuit < b->circuits.size() ? b->circuits[circuit].num_channels : 0;
+}
+
+int qsim_traj_plan(void* batch, const unsigned* circuit_of,
+                   const uint64_t* reps, unsigned num_traj,
+                   unsigned max_fused_size) {
+  auto* b = static_cast<TrajBatchHost*>(batch);
+  if (b == nullptr || circuit_of == nullptr || reps == nullptr ||
+      num_traj == 0 || max_fused_size < 1 || max_fused_size > 4) {
+    return QSIM_HIP_ERR_ARG;
+  }
+  for (unsigned t = 0; t < num_traj; ++t) {
+    if (circuit_of[t] >= b->circuits.size()) return QSIM_HIP_ERR_ARG;
+  }
+  if (b->handle) {
+    qsim_hip_traj_free(b->handle);
+    b->handle = nullptr;
+  }
+  b->have_results = false;
+  b->plans.assign(num_traj, TrajPlan{});
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int t = 0; t < int(num_traj); ++t) {
+    b->plans[t].circuit = circuit_of[t];
+    b->plans[t].rep = reps[t];
+    PlanTrajectory(b->circuits[circuit_of[t]], reps[t], max_fused_size,
+                   b->plans[t]);
+  }
+  // Device order: largest states first.
+  b->order.resize(num_traj);
+  for (unsigned t = 0; t < num_traj; ++t) b->order[t] = t;
+  std::stable_sort(b->order.begin(), b->order.end(), [&](unsigned x, unsigned y) {
+    return b->circuits[b->plans[x].circuit].num_qubits >
+           b->circuits[b->plans[y].circuit].num_qubits;
+  });
+  // Concatenate streams; identical matrices share one copy.
+  std::vector<float> pool(b->damp_k);
+  std::unordered_map<std::string, uint32_t> seen;
+  std::vector<qsim_hip_traj_op> ops;
+  std::vector<qsim_hip_traj_desc> descs;
+  std::vector<uint32_t> pairs;
+  b->pair_off.assign(num_traj, 0);
+  b->dec_off.assign(num_traj, 0);
+  uint32_t npairs = 0, ndec = 0;
+  for (unsigned t = 0; t < num_traj; ++t) {
+    b->pair_off[t] = npairs;
+    b->dec_off[t] = ndec;
+    const auto& nc = b->circuits[b->plans[t].circuit];
+    pairs.insert(pairs.end(), nc.pairs.begin(), nc.pairs.end());
+    npairs += nc.pairs.size() / 2;
+    ndec += b->plans[t].num_decisions;
+  }
+  for (unsigned t : b->order) {
+    TrajPlan& pl = b->plans[t];
+    const auto& nc = b->circuits[pl.circuit];
+    qsim_hip_traj_desc d{};
+    d.num_qubits = nc.num_qubits;
+    d.num_pairs = nc.pairs.size() / 2;
+    d.pair_offset = b->pair_off[t];
+    d.pad = pl.normalize_last ? 1 : 0;
+    d.op_begin = ops.size();
+    for (size_t i = 0; i < pl.ops.size(); ++i) {
+      qsim_hip_traj_op o = pl.ops[i];
+      if (o.kind == QSIM_HIP_TRAJ_GATE) {
+        const auto& m = pl.mats[i];
+        std::string key(reinterpret_cast<const char*>(m.data()),
+                        m.size() * sizeof(float));
+        auto it = seen.find(key);
+        if (it == seen.end()) {
+          it = seen.emplace(key, uint32_t(pool.size())).first;
+          pool.insert(pool.end(), m.begin(), m.end());
+        }
+        o.moff = it->second;
+      } else {
+        o.moff = 0;
+        o.moff2 = 8;
+        o.slot += b->dec_off[t];
+      }
+      ops.push_back(o);
+    }
+    d.op_end = ops.size();
+    descs.push_back(d);
+    pl.ops.clear();
+    pl.mats.clear();
+  }
+  b->total_pairs = npairs;
+  b->total_decisions = ndec;
+  return qsim_hip_traj_prepare(descs.data(), descs.size(), ops.data(),
+                               ops.size(), pool.data(), pool.size(),
+                               pairs.data(), npairs, ndec, &b->handle);
+}
+
+int qsim_traj_run(void* batch, unsigned threads, int use_graph) {
+  auto* b = static_cast<TrajBatchHost*>(batch);
+  if (b == nullptr || b->handle == nullptr) return QSIM_HIP_ERR_ARG;
+  b->have_results = false;
+  return qsim_hip_traj_run(b->handle, threads, use_graph);
+}
+
+static int Fetch(TrajBatchHost* b) {
+  if (b->have_results) return QSIM_HIP_OK;
+  b->zz.assign(b->total_pairs, 0.0);
+  b->dec.assign(b->total_decisions, -1);
+  int rc = qsim_hip_traj_results(b->handle, b->zz.data(), b->total_pairs,
+                                 b->dec.data(), b->total_decisions);
+  if (rc == QSIM_HIP_OK) b->have_results = true;
+  return rc;
+}
+
+int qsim_traj_zz(void* batch, unsigned t, double* zz, unsigned max,
+                 unsigned* num) {
+  auto* b = static_cast<TrajBatchHost*>(batch);
+  if (b == nullptr || b->handle == nullptr || t >= b->plans.size()) {
+    return QSIM_HIP_ERR_ARG;
+  }
+  int rc = Fetch(b);
+  if (rc != QSIM_HIP_OK) return rc;
+  const unsigned np = b->circuits[b->plans[t].circuit].pairs.size() / 2;
+  for (unsigned i = 0; i < np && i < max; ++i) zz[i] = b->zz[b->pair_off[t] + i];
+  if (num) *num = np;
+  return QSIM_HIP_OK;
+}
+
+int qsim_traj_kraus(void* batch, unsigned t, int32_t* kidx, unsigned max,
+                    unsigned* num) {
+  auto* b = static_cast<TrajBatchHost*>(batch);
+  if (b == nullptr || b->handle == nullptr || t >= b->plans.size()) {
+    return QSIM_HIP_ERR_ARG;
+  }
+  int rc = Fetch(b);
+  if (rc != QSIM_HIP_OK) return rc;
+  const auto& k = b->plans[t].kraus;
+  for (size_t i = 0; i < k.size() && i < max; ++i) {
+    kidx[i] = k[i] >= 0 ? k[i] : b->dec[b->dec_off[t] + uint32_t(-2 - k[i])];
+  }
+  if (num) *num = k.size();
+  return QSIM_HIP_OK;
+}
+
+int qsim_traj_state(void* batch, unsigned t, float* out) {
+  auto* b = static_cast<TrajBatchHost*>(batch);
+  if (b == nullptr || b->handle == nullptr || t >= b->plans.size()) {
+    return QSIM_HIP_ERR_ARG;
+  }
+  for (unsigned d = 0; d < b->order.size(); ++d) {
+    if (b->order[d] == t) return qsim_hip_traj_state(b->handle, d, out);
+  }
+  return QSIM_HIP_ERR_ARG;
+}
+
+void qsim_traj_free(void* batch) {
+  auto* b = static_cast<TrajBatchHost*>(batch);
+  if (b == nullptr) return;
+  if (b->handle) qsim_hip_traj_free(b->handle);
+  delete b;
+}
+
+}  // extern "C"

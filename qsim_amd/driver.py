@@ -58,6 +58,15 @@ def driver_lib():
                 ("qsim_driver_plan_learned", i, [vp, vp, u, vp, vp, vp, vp,
                                                  vp, vp, up, vp]),
                 ("qsim_driver_set_cost_model", None, [vp]),
+                ("qsim_traj_create", vp, [vp, u, up, up, ctypes.c_double,
+                                          ctypes.c_double]),
+                ("qsim_traj_num_channels", u, [vp, u]),
+                ("qsim_traj_plan", i, [vp, up, vp, u, u]),
+                ("qsim_traj_run", i, [vp, u, i]),
+                ("qsim_traj_zz", i, [vp, u, vp, u, up]),
+                ("qsim_traj_kraus", i, [vp, u, vp, u, up]),
+                ("qsim_traj_state", i, [vp, u, vp]),
+                ("qsim_traj_free", None, [vp]),
                 ("qsim_driver_run_passes", i, [vp, vp, i, u, u, u]),
                 ("qsim_driver_run", i, [vp, vp, i, u])):
             f = getattr(d, name)

@@ -1,11 +1,11 @@
 """Cache-blocked passes (qsim_hip_apply_gate_block) against the CPU oracle.
 
 A blocked pass applies a run of up to 32 gates (k <= 4, all inside one tile
-qubit set) with one stream of the state through HBM, in split-BF16x3 MFMA
-arithmetic.  The oracle applies the same gates one by one in FP64.  Bar: the
-complex64 result within 1e-4 relative L2 of the FP64 result after up to 32
-gates (about 1e-5 per gate from the 2^-16 products), i.e. fidelity above
-1 - 1e-8, the same bar per gate as the k >= 5 BF16x3 gate passes.
+qubit set) with one stream of the state through HBM, in split-BF16 MFMA
+arithmetic (matrix in three bf16 parts, amplitudes in two).  The oracle
+applies the same gates one by one in FP64.  Bar: the complex64 result within
+1e-4 relative L2 of the FP64 result after up to 32 gates (the amplitude
+split leaves about 2^-18 relative per gate), i.e. fidelity above 1 - 1e-8.
 """
 
 import numpy as np
