@@ -287,6 +287,74 @@ def run_cpu_baseline(args, circuit_file):
     }
 
 
+def run_trajectories(args):
+    """The batched noisy-trajectory engine on its own workload: 1024
+    trajectories of the grid circuit at 18, 20 and 22 qubits (341-342 each),
+    12 cycles, depolarising p = 1e-3 after every gate and amplitude damping
+    gamma = 5e-3 per qubit per cycle; one launch per batch, graph-replayed.
+    Reported beside the headline number, not as it."""
+    from qsim_amd import trajectories
+    sizes = [18, 20, 22]
+    ntraj = args.trajectories
+    b = trajectories.TrajectoryBatch(sizes, 12, p_depol=1e-3, gamma_damp=5e-3)
+    circuit_of = [i % 3 for i in range(ntraj)]
+    reps = list(range(ntraj))
+    plan_s = b.plan(circuit_of, reps, 4)
+    threads, kmax, tune = b.autotune(sample=96, log=log)
+    b.run(threads, use_graph=True)
+    times = []
+    for _ in range(3):
+        t0 = time.time()
+        b.run(threads, use_graph=True)
+        times.append(time.time() - t0)
+    best = min(times)
+    zz_mean = float(np.mean([np.mean(b.zz(t)) for t in range(ntraj)]))
+    log(f"trajectories: {ntraj} in {best * 1e3:.1f} ms "
+        f"({ntraj / best:.0f}/s), threads {threads}, k {kmax}, host plan "
+        f"{plan_s:.2f}s")
+    out = {"workload": "grid circuit 18/20/22 qubits, 12 cycles, "
+                       "depolarise 1e-3 + amplitude damping 5e-3",
+           "trajectories": ntraj, "batch_ms": best * 1e3,
+           "trajectories_per_s": ntraj / best,
+           "trajectories_per_s_with_host_planning": ntraj / (best + plan_s),
+           "threads": threads,
+           "max_fused_size": kmax, "host_plan_s": plan_s,
+           "autotune": tune, "mean_zz": zz_mean}
+    b.free()
+    if not args.no_cpu:
+        try:
+            out["cpu_baseline"] = trajectories_cpu(args, sizes)
+        except Exception as e:  # reported, never silently replaced
+            out["cpu_baseline"] = {"value": None, "error": repr(e)}
+    return out
+
+
+def trajectories_cpu(args, sizes):
+    """qsim's QuantumTrajectorySimulator (oracle/_ref) on a sample of the
+    batch: 4 trajectories per size, scaled to the batch's mix."""
+    import oracle
+    from qsim_amd import circuits
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    per = 4
+    total = 0.0
+    with tempfile.TemporaryDirectory() as tmp:
+        for n in sizes:
+            f = os.path.join(tmp, f"g{n}.txt")
+            with open(f, "w") as fh:
+                fh.write(circuits.grid_circuit(n, 12, 0))
+            rows, cols = circuits.grid_shape(n)
+            _, s = oracle.run_reference_trajectories(
+                f, rows, cols, 0, per, 1e-3, 5e-3, max_fused=4,
+                threads=threads)
+            total += s["sim_s"] / per
+    per_traj = total / len(sizes)
+    return {"value": 1.0 / per_traj, "unit": "trajectories/s",
+            "cores": threads, "kind": "reference",
+            "sample": f"{per} trajectories per size on qsim's "
+                      f"QuantumTrajectorySimulator + SimulatorAVX, "
+                      f"{threads} OpenMP threads"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -303,6 +371,13 @@ def main():
     ap.add_argument("--cpu-passes", type=int, default=6)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--pmc-calibration", action="store_true",
+                    help="only run two state-scaling passes over a 30-qubit "
+                         "complex64 state (8 GiB read + 8 GiB written each): "
+                         "the known-size reference for HBM counters")
+    ap.add_argument("--trajectories", type=int, default=1024,
+                    help="size of the trajectory batch reported beside the "
+                         "headline (0: skip)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -315,6 +390,16 @@ def main():
     from qsim_amd import circuits
     name, mem, cus = qsim_amd.device_info()
     log(f"device: {name}, {mem / 2**30:.0f} GiB, {cus} CUs")
+    if args.pmc_calibration:
+        ss = qsim_amd.StateSpace(np.float32)
+        st = ss.create(30)
+        ss.set_state_uniform(st)
+        for _ in range(2):
+            ss.multiply(1.0, st)    # ScaleKernel: 16 * 2^30 bytes each
+        qsim_amd.synchronize()
+        st.free()
+        log("pmc calibration: 2 ScaleKernel passes over 2^30 amplitudes")
+        return
     text = circuits.grid_circuit(args.qubits, args.cycles, args.seed)
     rows, cols = circuits.grid_shape(args.qubits)
     names = [p for p in args.paths.split(",") if p] or available_paths()
@@ -344,6 +429,14 @@ def main():
             except Exception as e:  # reported, never silently replaced
                 log(f"cpu baseline failed: {e!r}")
                 cpu = {"value": None, "error": repr(e)}
+
+    traj = None
+    if args.trajectories > 0:
+        try:
+            traj = run_trajectories(args)
+        except Exception as e:  # reported beside the headline, never hidden
+            log(f"trajectory batch failed: {e!r}")
+            traj = {"error": repr(e)}
 
     plain = results.get("plain-fp32-vector")
     out = {
@@ -379,6 +472,7 @@ def main():
                   for p, r in results.items()},
         "roofline": b["roofline"],
         "cpu_baseline": cpu,
+        "trajectory_batch": traj,
     }
     print(json.dumps(out), flush=True)
 

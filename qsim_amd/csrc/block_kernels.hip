@@ -41,6 +41,10 @@ typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
 constexpr int kThreads = 512;
 constexpr int kWaves = kThreads / 64;
+constexpr int kPipeThreads = 512;
+constexpr int kPipeWaves = kPipeThreads / 64;
+constexpr unsigned kPipeMaxOps = 16;  // 16 x 6 KiB of fragments beside the tile
+constexpr size_t kTileBytes = (size_t{1} << kBlockMaxBits) * sizeof(float2);
 // BF16 fragment elements of one 4-qubit op: 2 K blocks x (hi, mid, lo) x 512.
 constexpr size_t kOpFrag = 2 * 3 * 512;
 
@@ -49,6 +53,68 @@ __device__ __forceinline__ __bf16 Lo(float x) {
   return (__bf16)(x - (float)(__bf16)x);
 }
 
+// One 4-qubit op over the LDS tile: wave `wave` of `nwaves` takes batches
+// of 32 groups; `f` holds the op's fragments (hi, mid, lo per K block).
+__device__ __forceinline__ void ApplyTileOp(float2* lds, const uint8_t* tp,
+                                            const v8bf (&f)[6], unsigned nbatch,
+                                            int wave, int nwaves) {
+  const int l = threadIdx.x & 63;
+  const int j = l & 31, h = l >> 5;
+  const unsigned t0 = tp[0], t1 = tp[1], t2 = tp[2], t3 = tp[3];
+  auto toff = [&](unsigned m) {
+    return ((m & 1u) << t0) | (((m >> 1) & 1u) << t1) |
+           (((m >> 2) & 1u) << t2) | (((m >> 3) & 1u) << t3);
+  };
+  for (unsigned bt = wave; bt < nbatch; bt += nwaves) {
+    unsigned p0 = bt * 32 + j;
+    p0 = ((p0 >> t0) << (t0 + 1)) | (p0 & ((1u << t0) - 1));
+    p0 = ((p0 >> t1) << (t1 + 1)) | (p0 & ((1u << t1) - 1));
+    p0 = ((p0 >> t2) << (t2 + 1)) | (p0 & ((1u << t2) - 1));
+    p0 = ((p0 >> t3) << (t3 + 1)) | (p0 & ((1u << t3) - 1));
+    // Lane half h supplies members 8 kb + 4 h + 0..3 of group j.
+    float2 v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const unsigned m = 8 * (q >> 2) + 4 * h + (q & 3);
+      v[q] = lds[TileSwz(p0 + toff(m))];
+    }
+    v8bf bh0, bl0, bh1, bl1;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      bh0[2 * q] = Hi(v[q].x);
+      bl0[2 * q] = Lo(v[q].x);
+      bh0[2 * q + 1] = Hi(v[q].y);
+      bl0[2 * q + 1] = Lo(v[q].y);
+      bh1[2 * q] = Hi(v[4 + q].x);
+      bl1[2 * q] = Lo(v[4 + q].x);
+      bh1[2 * q + 1] = Hi(v[4 + q].y);
+      bl1[2 * q + 1] = Lo(v[4 + q].y);
+    }
+    // Two accumulators (one per K block) halve the dependent MFMA chain.
+    v16f c = v16f{}, d = v16f{};
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[2], bh0, c, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[5], bh1, d, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[1], bh0, c, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[4], bh1, d, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[0], bl0, c, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[3], bl1, d, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[0], bh0, c, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[3], bh1, d, 0, 0, 0);
+    c += d;
+    // Lane l holds (re, im) of members 4 s + 2 h + {0, 1}.
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const unsigned m = 4 * q + 2 * h + u;
+        lds[TileSwz(p0 + toff(m))] =
+            float2{c[4 * q + 2 * u], c[4 * q + 2 * u + 1]};
+      }
+  }
+}
+
+// Programs of more than kPipeMaxOps gates: fragments read from global memory
+// per op, two 512-thread workgroups per CU.
 __global__ void __launch_bounds__(kThreads)
 __attribute__((amdgpu_waves_per_eu(4)))
 GateBlockKernel(float2* __restrict__ s, BlockGeom g, BlockProgram p,
@@ -56,7 +122,6 @@ GateBlockKernel(float2* __restrict__ s, BlockGeom g, BlockProgram p,
   extern __shared__ char lds_raw[];
   float2* lds = reinterpret_cast<float2*>(lds_raw);
   const int l = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int j = l & 31, h = l >> 5;
   const unsigned nbatch = (1u << g.tb) >> 9;  // 32 groups of 16 per batch
   constexpr int PER = (1 << kBlockMaxBits) / kThreads;
   for (uint64_t tile = blockIdx.x; tile < g.ntiles; tile += gridDim.x) {
@@ -64,63 +129,76 @@ GateBlockKernel(float2* __restrict__ s, BlockGeom g, BlockProgram p,
     TileLoad<PER, kThreads, 8>(s, base, g, lds);
     __syncthreads();
     for (unsigned op = 0; op < p.nops; ++op) {
-      const unsigned t0 = p.tpos[op][0], t1 = p.tpos[op][1],
-                     t2 = p.tpos[op][2], t3 = p.tpos[op][3];
-      const v8bf* f = reinterpret_cast<const v8bf*>(frag + op * kOpFrag);
-      const v8bf ah0 = f[0 * 64 + l], am0 = f[1 * 64 + l], al0 = f[2 * 64 + l];
-      const v8bf ah1 = f[3 * 64 + l], am1 = f[4 * 64 + l], al1 = f[5 * 64 + l];
-      auto toff = [&](unsigned m) {
-        return ((m & 1u) << t0) | (((m >> 1) & 1u) << t1) |
-               (((m >> 2) & 1u) << t2) | (((m >> 3) & 1u) << t3);
-      };
-      for (unsigned bt = wave; bt < nbatch; bt += kWaves) {
-        unsigned p0 = bt * 32 + j;
-        p0 = ((p0 >> t0) << (t0 + 1)) | (p0 & ((1u << t0) - 1));
-        p0 = ((p0 >> t1) << (t1 + 1)) | (p0 & ((1u << t1) - 1));
-        p0 = ((p0 >> t2) << (t2 + 1)) | (p0 & ((1u << t2) - 1));
-        p0 = ((p0 >> t3) << (t3 + 1)) | (p0 & ((1u << t3) - 1));
-        // Lane half h supplies members 8 kb + 4 h + 0..3 of group j.
-        float2 v[8];
+      const v8bf* fp = reinterpret_cast<const v8bf*>(frag + op * kOpFrag);
+      v8bf f[6];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          const unsigned m = 8 * (q >> 2) + 4 * h + (q & 3);
-          v[q] = lds[TileSwz(p0 + toff(m))];
-        }
-        v8bf bh0, bl0, bh1, bl1;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          bh0[2 * q] = Hi(v[q].x);
-          bl0[2 * q] = Lo(v[q].x);
-          bh0[2 * q + 1] = Hi(v[q].y);
-          bl0[2 * q + 1] = Lo(v[q].y);
-          bh1[2 * q] = Hi(v[4 + q].x);
-          bl1[2 * q] = Lo(v[4 + q].x);
-          bh1[2 * q + 1] = Hi(v[4 + q].y);
-          bl1[2 * q + 1] = Lo(v[4 + q].y);
-        }
-        v16f c = v16f{};
-        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al0, bh0, c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(am0, bh0, c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah0, bl0, c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah0, bh0, c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al1, bh1, c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(am1, bh1, c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah1, bl1, c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah1, bh1, c, 0, 0, 0);
-        // Lane l holds (re, im) of members 4 s + 2 h + {0, 1}.
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-          for (int u = 0; u < 2; ++u) {
-            const unsigned m = 4 * q + 2 * h + u;
-            lds[TileSwz(p0 + toff(m))] =
-                float2{c[4 * q + 2 * u], c[4 * q + 2 * u + 1]};
-          }
-      }
+      for (int r = 0; r < 6; ++r) f[r] = fp[r * 64 + l];
+      ApplyTileOp(lds, p.tpos[op], f, nbatch, wave, kWaves);
       __syncthreads();
     }
     TileStore<PER, kThreads, 8>(s, base, g, lds);
     __syncthreads();
+  }
+}
+
+// Programs of up to kPipeMaxOps gates: one 1024-thread workgroup per CU
+// keeps the program's fragments in LDS beside the tile and software-
+// pipelines the tiles: the next tile's loads are issued into registers
+// before the current tile's gates run, so HBM latency hides behind the
+// MFMA work.  (No global loads happen between the prefetch and its use, so
+// the in-order vector-memory counter never makes a gate wait for it.)
+__global__ void __launch_bounds__(kPipeThreads)
+__attribute__((amdgpu_waves_per_eu(2)))
+GateBlockPipeKernel(float2* __restrict__ s, BlockGeom g, BlockProgram p,
+                    const uint16_t* __restrict__ frag) {
+  extern __shared__ char lds_raw[];
+  float2* lds = reinterpret_cast<float2*>(lds_raw);
+  v8bf* lfrag = reinterpret_cast<v8bf*>(lds_raw + kTileBytes);
+  const int l = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const unsigned T = 1u << g.tb;
+  const unsigned nbatch = T >> 9;
+  constexpr int PER = (1 << kBlockMaxBits) / kPipeThreads;
+
+  const unsigned nfv = p.nops * unsigned(kOpFrag / 8);
+  const v8bf* gf = reinterpret_cast<const v8bf*>(frag);
+  for (unsigned i = threadIdx.x; i < nfv; i += kPipeThreads) lfrag[i] = gf[i];
+
+  float2 r[PER];
+  auto prefetch = [&](uint64_t t) {
+    const uint64_t b = InsertZeros(t << g.Lc, g.outer, g.nouter);
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const unsigned a = threadIdx.x + kPipeThreads * i;
+      if (a < T) r[i] = s[TileAddr(b, a, g)];
+    }
+  };
+  uint64_t tile = blockIdx.x;
+  if (tile < g.ntiles) prefetch(tile);
+  while (tile < g.ntiles) {
+    const uint64_t base = InsertZeros(tile << g.Lc, g.outer, g.nouter);
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const unsigned a = threadIdx.x + kPipeThreads * i;
+      if (a < T) lds[TileSwz(a)] = r[i];
+    }
+    __syncthreads();
+    const uint64_t next = tile + gridDim.x;
+    if (next < g.ntiles) prefetch(next);
+    for (unsigned op = 0; op < p.nops; ++op) {
+      const v8bf* fp = lfrag + op * unsigned(kOpFrag / 8);
+      v8bf f[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) f[k] = fp[k * 64 + l];
+      ApplyTileOp(lds, p.tpos[op], f, nbatch, wave, kPipeWaves);
+      __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const unsigned a = threadIdx.x + kPipeThreads * i;
+      if (a < T) s[TileAddr(base, a, g)] = lds[TileSwz(a)];
+    }
+    __syncthreads();
+    tile = next;
   }
 }
 
@@ -247,6 +325,25 @@ int PrepareGateBlock(unsigned n, const unsigned* tile, unsigned nt,
 
 int RunGateBlock(float2* s, const BlockGeom& g, const BlockProgram& p,
                  const uint16_t* dfrag) {
+  if (p.nops <= kPipeMaxOps) {
+    static bool attr = false;
+    const size_t lds = kTileBytes + kPipeMaxOps * kOpFrag * sizeof(uint16_t);
+    if (!attr) {
+      QHIP_CHECK(hipFuncSetAttribute(
+          reinterpret_cast<const void*>(&GateBlockPipeKernel),
+          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+      attr = true;
+    }
+    // One workgroup per CU, each streaming its share of the tiles.
+    const uint64_t blocks = std::min<uint64_t>(g.ntiles, uint64_t(NumCUs()));
+    const size_t used = kTileBytes + p.nops * kOpFrag * sizeof(uint16_t);
+    PassTimerBegin();
+    hipLaunchKernelGGL(GateBlockPipeKernel, dim3(unsigned(blocks)),
+                       dim3(kPipeThreads), used, Stream(), s, g, p, dfrag);
+    PassTimerEnd();
+    QHIP_LAUNCHED("GateBlockPipeKernel");
+    return QSIM_HIP_OK;
+  }
   const size_t lds = (size_t{1} << g.tb) * sizeof(float2);
   // Two workgroups per CU fit in LDS; a few tiles each.
   const uint64_t blocks =
