@@ -168,6 +168,94 @@ class Frontier {
   std::vector<size_t> ptr_;
 };
 
+// A fused gate of a blocked pass: ascending qubits, FP64 row-major
+// interleaved matrix (bit j of an index <-> qubits[j]).
+struct TileGate {
+  std::vector<unsigned> qubits;
+  std::vector<double> matrix;
+};
+
+// m over `qs` (ascending) embedded in the space of `u` (ascending superset),
+// identity on the other qubits of u.
+inline std::vector<double> Embed(const std::vector<double>& m,
+                                 const std::vector<unsigned>& qs,
+                                 const std::vector<unsigned>& u) {
+  const unsigned D = 1u << u.size(), d = 1u << qs.size();
+  unsigned pos[4] = {0, 0, 0, 0}, mask = 0;
+  for (size_t i = 0; i < qs.size(); ++i) {
+    for (size_t j = 0; j < u.size(); ++j)
+      if (u[j] == qs[i]) pos[i] = j;
+    mask |= 1u << pos[i];
+  }
+  std::vector<double> out(2 * size_t(D) * D, 0.0);
+  for (unsigned r = 0; r < D; ++r) {
+    for (unsigned c = 0; c < D; ++c) {
+      if ((r & ~mask) != (c & ~mask)) continue;
+      unsigned mr = 0, mc = 0;
+      for (size_t i = 0; i < qs.size(); ++i) {
+        mr |= ((r >> pos[i]) & 1u) << i;
+        mc |= ((c >> pos[i]) & 1u) << i;
+      }
+      out[2 * (size_t(r) * D + c)] = m[2 * (size_t(mr) * d + mc)];
+      out[2 * (size_t(r) * D + c) + 1] = m[2 * (size_t(mr) * d + mc) + 1];
+    }
+  }
+  return out;
+}
+
+// Every fused gate of a blocked pass costs one MFMA sweep of the tile
+// whatever its size (<= 4 qubits), so gates are merged while the union of
+// their qubits stays within 4: gate b joins an earlier gate a when no gate
+// between them touches b's qubits (b commutes back to a), as the product
+// U_b U_a on the union.
+inline void MergeTileGates(std::vector<TileGate>& g) {
+  auto overlaps = [](const std::vector<unsigned>& x,
+                     const std::vector<unsigned>& y) {
+    for (unsigned a : x)
+      for (unsigned b : y)
+        if (a == b) return true;
+    return false;
+  };
+  bool changed = true;
+  while (changed) {
+    changed = false;
+    for (size_t a = 0; a < g.size() && !changed; ++a) {
+      for (size_t b = a + 1; b < g.size(); ++b) {
+        std::vector<unsigned> u = g[a].qubits;
+        for (unsigned q : g[b].qubits)
+          if (std::find(u.begin(), u.end(), q) == u.end()) u.push_back(q);
+        if (u.size() > 4) continue;
+        bool blocked = false;
+        for (size_t k = a + 1; k < b && !blocked; ++k)
+          blocked = overlaps(g[k].qubits, g[b].qubits);
+        if (blocked) continue;
+        std::sort(u.begin(), u.end());
+        const auto ma = Embed(g[a].matrix, g[a].qubits, u);
+        const auto mb = Embed(g[b].matrix, g[b].qubits, u);
+        const unsigned D = 1u << u.size();
+        std::vector<double> prod(2 * size_t(D) * D, 0.0);
+        for (unsigned r = 0; r < D; ++r)
+          for (unsigned k = 0; k < D; ++k) {
+            const double br = mb[2 * (size_t(r) * D + k)];
+            const double bi = mb[2 * (size_t(r) * D + k) + 1];
+            if (br == 0 && bi == 0) continue;
+            for (unsigned c = 0; c < D; ++c) {
+              const double ar = ma[2 * (size_t(k) * D + c)];
+              const double ai = ma[2 * (size_t(k) * D + c) + 1];
+              prod[2 * (size_t(r) * D + c)] += br * ar - bi * ai;
+              prod[2 * (size_t(r) * D + c) + 1] += br * ai + bi * ar;
+            }
+          }
+        g[a].qubits = u;
+        g[a].matrix = std::move(prod);
+        g.erase(g.begin() + b);
+        changed = true;
+        break;
+      }
+    }
+  }
+}
+
 }  // namespace detail
 
 // Plans blocked passes for ops of a circuit on `num_qubits` qubits.  Returns
@@ -203,6 +291,15 @@ bool PlanBlocks(const BlockPlannerParameter& param, unsigned num_qubits,
         auto fused =
             MultiQubitGateFuser<IO>::template FuseGates<const Operation<FP>*>(
                 fp, num_qubits, ptrs.cbegin(), ptrs.cend(), no_splits, true);
+        std::vector<detail::TileGate> tg;
+        for (const auto& f : fused) {
+          const auto* fg = std::get_if<FusedGate<FP>>(&f);
+          if (fg == nullptr) return false;  // only gates were passed in
+          tg.push_back({fg->qubits,
+                        std::vector<double>(fg->matrix.begin(),
+                                            fg->matrix.end())});
+        }
+        detail::MergeTileGates(tg);
         BlockPass bp;
         bp.tile = tile;
         bp.num_gates = taken.size();
@@ -214,14 +311,12 @@ bool PlanBlocks(const BlockPlannerParameter& param, unsigned num_qubits,
           bp.m.clear();
           bp.num_gates = 0;
         };
-        for (const auto& f : fused) {
-          const auto* fg = std::get_if<FusedGate<FP>>(&f);
-          if (fg == nullptr) return false;  // only gates were passed in
+        for (const auto& t : tg) {
           if (bp.ks.size() == param.max_ops) flush();
-          bp.ks.push_back(fg->qubits.size());
+          bp.ks.push_back(t.qubits.size());
           for (unsigned b = 0; b < 4; ++b)
-            bp.qs.push_back(b < fg->qubits.size() ? fg->qubits[b] : 0);
-          for (auto x : fg->matrix) bp.m.push_back(double(x));
+            bp.qs.push_back(b < t.qubits.size() ? t.qubits[b] : 0);
+          bp.m.insert(bp.m.end(), t.matrix.begin(), t.matrix.end());
         }
         if (!bp.ks.empty()) flush();
       }

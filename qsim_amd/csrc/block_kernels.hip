@@ -20,8 +20,9 @@
 // C register r of lane l = C[8(r/4) + 4(l/32) + r%4][l%32].
 //
 // The grid is a tile-stride loop; 512 threads (8 waves) share a tile, and
-// the 64 KiB tile lets two workgroups share a CU's 160 KiB of LDS, so one
-// workgroup's HBM phase overlaps the other's MFMA phase.
+// the 64 KiB tile (plus a 12 KiB fragment ring) lets two workgroups share a
+// CU's 160 KiB of LDS, so one workgroup's HBM phase overlaps the other's
+// MFMA phase.
 
 #include <hip/hip_runtime.h>
 
@@ -38,20 +39,18 @@ namespace {
 
 typedef float v16f __attribute__((ext_vector_type(16)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+typedef float v2f __attribute__((ext_vector_type(2)));
 
 constexpr int kThreads = 512;
 constexpr int kWaves = kThreads / 64;
-constexpr int kPipeThreads = 512;
-constexpr int kPipeWaves = kPipeThreads / 64;
-constexpr unsigned kPipeMaxOps = 16;  // 16 x 6 KiB of fragments beside the tile
 constexpr size_t kTileBytes = (size_t{1} << kBlockMaxBits) * sizeof(float2);
 // BF16 fragment elements of one 4-qubit op: 2 K blocks x (hi, mid, lo) x 512.
 constexpr size_t kOpFrag = 2 * 3 * 512;
-
-__device__ __forceinline__ __bf16 Hi(float x) { return (__bf16)x; }
-__device__ __forceinline__ __bf16 Lo(float x) {
-  return (__bf16)(x - (float)(__bf16)x);
-}
+constexpr int kOpWords = int(kOpFrag / 2);       // 32-bit words per op (6 KiB)
+constexpr int kRingWords = kOpWords / kThreads;  // per thread per op
+static_assert(kOpWords % kThreads == 0, "fragment ring split");
+constexpr size_t kBlockLds = kTileBytes + 2 * kOpWords * sizeof(uint32_t);
 
 // One 4-qubit op over the LDS tile: wave `wave` of `nwaves` takes batches
 // of 32 groups; `f` holds the op's fragments (hi, mid, lo per K block).
@@ -65,140 +64,189 @@ __device__ __forceinline__ void ApplyTileOp(float2* lds, const uint8_t* tp,
     return ((m & 1u) << t0) | (((m >> 1) & 1u) << t1) |
            (((m >> 2) & 1u) << t2) | (((m >> 3) & 1u) << t3);
   };
+  // TileSwz is linear over GF(2) and a member's offset has no bit in common
+  // with its group base, so slot(base + off) = TileSwz(base) ^ TileSwz(off):
+  // per member one XOR with a per-op constant (the lane half's part of the
+  // member index folded in once).
+  const unsigned sh = TileSwz(toff(4u * unsigned(h)));
+  unsigned rd[8], wr[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    rd[q] = sh ^ TileSwz(toff(8u * unsigned(q >> 2) + unsigned(q & 3)));
+  }
+  const unsigned shw = TileSwz(toff(2u * unsigned(h)));
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int u = 0; u < 2; ++u) wr[2 * q + u] = shw ^ TileSwz(toff(4u * q + u));
   for (unsigned bt = wave; bt < nbatch; bt += nwaves) {
     unsigned p0 = bt * 32 + j;
     p0 = ((p0 >> t0) << (t0 + 1)) | (p0 & ((1u << t0) - 1));
     p0 = ((p0 >> t1) << (t1 + 1)) | (p0 & ((1u << t1) - 1));
     p0 = ((p0 >> t2) << (t2 + 1)) | (p0 & ((1u << t2) - 1));
     p0 = ((p0 >> t3) << (t3 + 1)) | (p0 & ((1u << t3) - 1));
-    // Lane half h supplies members 8 kb + 4 h + 0..3 of group j.
-    float2 v[8];
+    const unsigned sp = TileSwz(p0);
+    // Lane half h supplies members 8 kb + 4 h + 0..3 of group j, split into
+    // bf16 (hi, lo) pairs two values at a time.
+    v8bf bh[2], bl[2];
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-      const unsigned m = 8 * (q >> 2) + 4 * h + (q & 3);
-      v[q] = lds[TileSwz(p0 + toff(m))];
+      const float2 v = lds[sp ^ rd[q]];
+      const v2f x = {v.x, v.y};
+      const v2bf hi = __builtin_convertvector(x, v2bf);
+      const v2f r = x - __builtin_convertvector(hi, v2f);
+      const v2bf lo = __builtin_convertvector(r, v2bf);
+      bh[q >> 2][2 * (q & 3)] = hi[0];
+      bh[q >> 2][2 * (q & 3) + 1] = hi[1];
+      bl[q >> 2][2 * (q & 3)] = lo[0];
+      bl[q >> 2][2 * (q & 3) + 1] = lo[1];
     }
-    v8bf bh0, bl0, bh1, bl1;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      bh0[2 * q] = Hi(v[q].x);
-      bl0[2 * q] = Lo(v[q].x);
-      bh0[2 * q + 1] = Hi(v[q].y);
-      bl0[2 * q + 1] = Lo(v[q].y);
-      bh1[2 * q] = Hi(v[4 + q].x);
-      bl1[2 * q] = Lo(v[4 + q].x);
-      bh1[2 * q + 1] = Hi(v[4 + q].y);
-      bl1[2 * q + 1] = Lo(v[4 + q].y);
-    }
-    // Two accumulators (one per K block) halve the dependent MFMA chain.
-    v16f c = v16f{}, d = v16f{};
-    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[2], bh0, c, 0, 0, 0);
-    d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[5], bh1, d, 0, 0, 0);
-    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[1], bh0, c, 0, 0, 0);
-    d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[4], bh1, d, 0, 0, 0);
-    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[0], bl0, c, 0, 0, 0);
-    d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[3], bl1, d, 0, 0, 0);
-    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[0], bh0, c, 0, 0, 0);
-    d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[3], bh1, d, 0, 0, 0);
-    c += d;
+    // One accumulator: the vector ALU, not MFMA latency, bounds this loop
+    // (other waves cover the dependent chain), so no extra adds.
+    v16f c = v16f{};
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[2], bh[0], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[1], bh[0], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[0], bl[0], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[0], bh[0], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[5], bh[1], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[4], bh[1], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[3], bl[1], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[3], bh[1], c, 0, 0, 0);
     // Lane l holds (re, im) of members 4 s + 2 h + {0, 1}.
 #pragma unroll
     for (int q = 0; q < 4; ++q)
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
-        const unsigned m = 4 * q + 2 * h + u;
-        lds[TileSwz(p0 + toff(m))] =
-            float2{c[4 * q + 2 * u], c[4 * q + 2 * u + 1]};
+        lds[sp ^ wr[2 * q + u]] = float2{c[4 * q + 2 * u], c[4 * q + 2 * u + 1]};
       }
   }
 }
 
-// Programs of more than kPipeMaxOps gates: fragments read from global memory
-// per op, two 512-thread workgroups per CU.
+// Offset (in amplitudes) of tile-local high index x: x's bits deposited at
+// the tile's high qubits.
+__device__ __forceinline__ uint64_t DepositHigh(unsigned x, const BlockGeom& g) {
+  uint64_t o = 0;
+  for (unsigned b = 0; b < g.h; ++b) o += ((x >> b) & 1u) ? g.hbit[b] : 0;
+  return o;
+}
+
+// Tile staging with addresses split into a per-thread part and a per-element
+// part.  Element i of thread t is tile-local amplitude a = t + NT i.  The
+// host keeps the low run within the thread bits (Lc <= log2 NT = 9), so a's
+// low bits come from t alone and its high index is
+// (t >> Lc) | (i << (9 - Lc)) with disjoint bits: its offset is
+// D(t >> Lc) + D(i << (9 - Lc)), one add per element instead of a deposit
+// loop.  The element parts are wave-uniform and kept in units of 2^Lc
+// amplitudes (32 bits: no tile qubit is 32 above the low run).
+template <int PER, int NT>
+struct TileIo {
+  static_assert(NT == 512, "thread bits");
+  static constexpr unsigned kLogNT = 9;
+  uint64_t tofs;       // this thread's part
+  uint32_t eofs[PER];  // the element parts >> Lc
+
+  __device__ __forceinline__ void Init(const BlockGeom& g) {
+    const unsigned t = threadIdx.x;
+    tofs = (t & ((1u << g.Lc) - 1)) + DepositHigh(t >> g.Lc, g);
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      eofs[i] = uint32_t(DepositHigh(unsigned(i) << (kLogNT - g.Lc), g) >> g.Lc);
+    }
+  }
+
+  template <int CH>
+  __device__ __forceinline__ void Load(const float2* __restrict__ s,
+                                       uint64_t base, const BlockGeom& g,
+                                       float2* lds) const {
+    const unsigned T = 1u << g.tb;
+    const float2* p = s + base + tofs;
+#pragma unroll
+    for (int i0 = 0; i0 < PER; i0 += CH) {
+      float2 v[CH];
+#pragma unroll
+      for (int i = 0; i < CH; ++i) {
+        const unsigned a = threadIdx.x + NT * (i0 + i);
+        if (a < T) v[i] = p[uint64_t(eofs[i0 + i]) << g.Lc];
+      }
+#pragma unroll
+      for (int i = 0; i < CH; ++i) {
+        const unsigned a = threadIdx.x + NT * (i0 + i);
+        if (a < T) lds[TileSwz(a)] = v[i];
+      }
+    }
+  }
+
+  template <int CH>
+  __device__ __forceinline__ void Store(float2* __restrict__ s, uint64_t base,
+                                        const BlockGeom& g,
+                                        const float2* lds) const {
+    const unsigned T = 1u << g.tb;
+    float2* p = s + base + tofs;
+#pragma unroll
+    for (int i0 = 0; i0 < PER; i0 += CH) {
+      float2 v[CH];
+#pragma unroll
+      for (int i = 0; i < CH; ++i) {
+        const unsigned a = threadIdx.x + NT * (i0 + i);
+        if (a < T) v[i] = lds[TileSwz(a)];
+      }
+#pragma unroll
+      for (int i = 0; i < CH; ++i) {
+        const unsigned a = threadIdx.x + NT * (i0 + i);
+        if (a < T) p[uint64_t(eofs[i0 + i]) << g.Lc] = v[i];
+      }
+    }
+  }
+};
+
+// Two 512-thread workgroups per CU, each with a 64 KiB tile and a two-slot
+// ring of op fragments in LDS: while op i runs from slot i % 2, each thread
+// holds its 12 bytes of op i + 1's fragments in flight from global memory
+// (L2) and writes them to the other slot before the op's barrier, so the
+// fragment latency hides behind the op's MFMA work.  The program repeats for
+// every tile, so the ring wraps from the last op to the first.
 __global__ void __launch_bounds__(kThreads)
 __attribute__((amdgpu_waves_per_eu(4)))
 GateBlockKernel(float2* __restrict__ s, BlockGeom g, BlockProgram p,
                 const uint16_t* __restrict__ frag) {
   extern __shared__ char lds_raw[];
   float2* lds = reinterpret_cast<float2*>(lds_raw);
+  uint32_t* ring = reinterpret_cast<uint32_t*>(lds_raw + kTileBytes);
   const int l = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const unsigned nbatch = (1u << g.tb) >> 9;  // 32 groups of 16 per batch
   constexpr int PER = (1 << kBlockMaxBits) / kThreads;
+  const uint32_t* gfrag = reinterpret_cast<const uint32_t*>(frag);
+  // Op 0's fragments into slot 0.
+#pragma unroll
+  for (int w = 0; w < kRingWords; ++w) {
+    ring[threadIdx.x + kThreads * w] = gfrag[threadIdx.x + kThreads * w];
+  }
+  TileIo<PER, kThreads> io;
+  io.Init(g);
+  unsigned k = 0;  // ops run so far: the current op's slot is k % 2
   for (uint64_t tile = blockIdx.x; tile < g.ntiles; tile += gridDim.x) {
     const uint64_t base = InsertZeros(tile << g.Lc, g.outer, g.nouter);
-    TileLoad<PER, kThreads, 8>(s, base, g, lds);
+    io.Load<8>(s, base, g, lds);
     __syncthreads();
-    for (unsigned op = 0; op < p.nops; ++op) {
-      const v8bf* fp = reinterpret_cast<const v8bf*>(frag + op * kOpFrag);
+    for (unsigned op = 0; op < p.nops; ++op, ++k) {
+      const unsigned nxt = op + 1 == p.nops ? 0 : op + 1;
+      const uint32_t* gn = gfrag + size_t(nxt) * kOpWords;
+      uint32_t pre[kRingWords];
+#pragma unroll
+      for (int w = 0; w < kRingWords; ++w) pre[w] = gn[threadIdx.x + kThreads * w];
+      const v8bf* fp =
+          reinterpret_cast<const v8bf*>(ring + (k & 1u) * kOpWords);
       v8bf f[6];
 #pragma unroll
       for (int r = 0; r < 6; ++r) f[r] = fp[r * 64 + l];
       ApplyTileOp(lds, p.tpos[op], f, nbatch, wave, kWaves);
+      uint32_t* slot = ring + ((k + 1) & 1u) * kOpWords;
+#pragma unroll
+      for (int w = 0; w < kRingWords; ++w) slot[threadIdx.x + kThreads * w] = pre[w];
       __syncthreads();
     }
-    TileStore<PER, kThreads, 8>(s, base, g, lds);
+    io.Store<8>(s, base, g, lds);
     __syncthreads();
-  }
-}
-
-// Programs of up to kPipeMaxOps gates: one 1024-thread workgroup per CU
-// keeps the program's fragments in LDS beside the tile and software-
-// pipelines the tiles: the next tile's loads are issued into registers
-// before the current tile's gates run, so HBM latency hides behind the
-// MFMA work.  (No global loads happen between the prefetch and its use, so
-// the in-order vector-memory counter never makes a gate wait for it.)
-__global__ void __launch_bounds__(kPipeThreads)
-__attribute__((amdgpu_waves_per_eu(2)))
-GateBlockPipeKernel(float2* __restrict__ s, BlockGeom g, BlockProgram p,
-                    const uint16_t* __restrict__ frag) {
-  extern __shared__ char lds_raw[];
-  float2* lds = reinterpret_cast<float2*>(lds_raw);
-  v8bf* lfrag = reinterpret_cast<v8bf*>(lds_raw + kTileBytes);
-  const int l = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const unsigned T = 1u << g.tb;
-  const unsigned nbatch = T >> 9;
-  constexpr int PER = (1 << kBlockMaxBits) / kPipeThreads;
-
-  const unsigned nfv = p.nops * unsigned(kOpFrag / 8);
-  const v8bf* gf = reinterpret_cast<const v8bf*>(frag);
-  for (unsigned i = threadIdx.x; i < nfv; i += kPipeThreads) lfrag[i] = gf[i];
-
-  float2 r[PER];
-  auto prefetch = [&](uint64_t t) {
-    const uint64_t b = InsertZeros(t << g.Lc, g.outer, g.nouter);
-#pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const unsigned a = threadIdx.x + kPipeThreads * i;
-      if (a < T) r[i] = s[TileAddr(b, a, g)];
-    }
-  };
-  uint64_t tile = blockIdx.x;
-  if (tile < g.ntiles) prefetch(tile);
-  while (tile < g.ntiles) {
-    const uint64_t base = InsertZeros(tile << g.Lc, g.outer, g.nouter);
-#pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const unsigned a = threadIdx.x + kPipeThreads * i;
-      if (a < T) lds[TileSwz(a)] = r[i];
-    }
-    __syncthreads();
-    const uint64_t next = tile + gridDim.x;
-    if (next < g.ntiles) prefetch(next);
-    for (unsigned op = 0; op < p.nops; ++op) {
-      const v8bf* fp = lfrag + op * unsigned(kOpFrag / 8);
-      v8bf f[6];
-#pragma unroll
-      for (int k = 0; k < 6; ++k) f[k] = fp[k * 64 + l];
-      ApplyTileOp(lds, p.tpos[op], f, nbatch, wave, kPipeWaves);
-      __syncthreads();
-    }
-#pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const unsigned a = threadIdx.x + kPipeThreads * i;
-      if (a < T) s[TileAddr(base, a, g)] = lds[TileSwz(a)];
-    }
-    __syncthreads();
-    tile = next;
   }
 }
 
@@ -266,6 +314,13 @@ int PrepareGateBlock(unsigned n, const unsigned* tile, unsigned nt,
              "and add at most %u higher ones", 2u, kBlockMaxHigh);
     return QSIM_HIP_ERR_ARG;
   }
+  // The kernel's staging keeps the contiguous run within its 9 thread bits;
+  // further low qubits are handled as tile qubits above the run (the same
+  // tile-local positions and addresses).
+  if (g.Lc > 9) {
+    g.Lc = 9;
+    g.h = nt - 9;
+  }
   g.nouter = g.h;
   for (unsigned b = 0; b < g.h; ++b) {
     g.outer[b] = tile[g.Lc + b];
@@ -325,32 +380,19 @@ int PrepareGateBlock(unsigned n, const unsigned* tile, unsigned nt,
 
 int RunGateBlock(float2* s, const BlockGeom& g, const BlockProgram& p,
                  const uint16_t* dfrag) {
-  if (p.nops <= kPipeMaxOps) {
-    static bool attr = false;
-    const size_t lds = kTileBytes + kPipeMaxOps * kOpFrag * sizeof(uint16_t);
-    if (!attr) {
-      QHIP_CHECK(hipFuncSetAttribute(
-          reinterpret_cast<const void*>(&GateBlockPipeKernel),
-          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-      attr = true;
-    }
-    // One workgroup per CU, each streaming its share of the tiles.
-    const uint64_t blocks = std::min<uint64_t>(g.ntiles, uint64_t(NumCUs()));
-    const size_t used = kTileBytes + p.nops * kOpFrag * sizeof(uint16_t);
-    PassTimerBegin();
-    hipLaunchKernelGGL(GateBlockPipeKernel, dim3(unsigned(blocks)),
-                       dim3(kPipeThreads), used, Stream(), s, g, p, dfrag);
-    PassTimerEnd();
-    QHIP_LAUNCHED("GateBlockPipeKernel");
-    return QSIM_HIP_OK;
+  static bool attr = false;
+  if (!attr) {
+    QHIP_CHECK(hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&GateBlockKernel),
+        hipFuncAttributeMaxDynamicSharedMemorySize, int(kBlockLds)));
+    attr = true;
   }
-  const size_t lds = (size_t{1} << g.tb) * sizeof(float2);
-  // Two workgroups per CU fit in LDS; a few tiles each.
+  // Two workgroups per CU fit in LDS (2 x 76 KiB); a few tiles each.
   const uint64_t blocks =
       std::min<uint64_t>(g.ntiles, uint64_t(NumCUs()) * 2 * 8);
   PassTimerBegin();
   hipLaunchKernelGGL(GateBlockKernel, dim3(unsigned(blocks)), dim3(kThreads),
-                     lds, Stream(), s, g, p, dfrag);
+                     kBlockLds, Stream(), s, g, p, dfrag);
   PassTimerEnd();
   QHIP_LAUNCHED("GateBlockKernel");
   return QSIM_HIP_OK;
