@@ -226,7 +226,7 @@ GateBlockKernel(float2* __restrict__ s, BlockGeom g, BlockProgram p,
   unsigned k = 0;  // ops run so far: the current op's slot is k % 2
   for (uint64_t tile = blockIdx.x; tile < g.ntiles; tile += gridDim.x) {
     const uint64_t base = InsertZeros(tile << g.Lc, g.outer, g.nouter);
-    io.Load<8>(s, base, g, lds);
+    io.Load<16>(s, base, g, lds);
     __syncthreads();
     for (unsigned op = 0; op < p.nops; ++op, ++k) {
       const unsigned nxt = op + 1 == p.nops ? 0 : op + 1;
@@ -245,7 +245,7 @@ GateBlockKernel(float2* __restrict__ s, BlockGeom g, BlockProgram p,
       for (int w = 0; w < kRingWords; ++w) slot[threadIdx.x + kThreads * w] = pre[w];
       __syncthreads();
     }
-    io.Store<8>(s, base, g, lds);
+    io.Store<16>(s, base, g, lds);
     __syncthreads();
   }
 }
