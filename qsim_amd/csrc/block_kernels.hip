@@ -28,6 +28,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "internal.h"
@@ -380,13 +381,15 @@ int PrepareGateBlock(unsigned n, const unsigned* tile, unsigned nt,
 
 int RunGateBlock(float2* s, const BlockGeom& g, const BlockProgram& p,
                  const uint16_t* dfrag) {
-  static bool attr = false;
-  if (!attr) {
-    QHIP_CHECK(hipFuncSetAttribute(
-        reinterpret_cast<const void*>(&GateBlockKernel),
-        hipFuncAttributeMaxDynamicSharedMemorySize, int(kBlockLds)));
-    attr = true;
-  }
+  // More than 64 KiB of dynamic LDS must be allowed once per process.
+  static std::once_flag once;
+  static hipError_t attr = hipSuccess;
+  std::call_once(once, [] {
+    attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&GateBlockKernel),
+                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                               int(kBlockLds));
+  });
+  QHIP_CHECK(attr);
   // Two workgroups per CU fit in LDS (2 x 76 KiB); a few tiles each.
   const uint64_t blocks =
       std::min<uint64_t>(g.ntiles, uint64_t(NumCUs()) * 2 * 8);
