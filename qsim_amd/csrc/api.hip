@@ -28,8 +28,8 @@ struct Device {
   // Matrix ring: device slots and pinned host staging, one event per slot.
   static constexpr int kSlots = 64;
   // A k = 6 FP64 matrix or BF16 fragments, or the fragments of a full
-  // cache-blocked program (32 x 6 KiB).
-  static constexpr size_t kSlotBytes = 192 * 1024;
+  // cache-blocked program (32 op records of 8 KiB).
+  static constexpr size_t kSlotBytes = 256 * 1024;
   char* mat_dev = nullptr;
   char* mat_host = nullptr;
   hipEvent_t mat_ev[kSlots] = {};

@@ -20,7 +20,7 @@
 // C register r of lane l = C[8(r/4) + 4(l/32) + r%4][l%32].
 //
 // The grid is a tile-stride loop; 512 threads (8 waves) share a tile, and
-// the 64 KiB tile (plus a 12 KiB fragment ring) lets two workgroups share a
+// the 64 KiB tile (plus a 16 KiB ring of op records) lets two workgroups share a
 // CU's 160 KiB of LDS, so one workgroup's HBM phase overlaps the other's
 // MFMA phase.
 
@@ -43,83 +43,130 @@ typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
 typedef float v2f __attribute__((ext_vector_type(2)));
 
-constexpr int kThreads = 512;
+constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 constexpr size_t kTileBytes = (size_t{1} << kBlockMaxBits) * sizeof(float2);
 // BF16 fragment elements of one 4-qubit op: 2 K blocks x (hi, mid, lo) x 512.
 constexpr size_t kOpFrag = 2 * 3 * 512;
-constexpr int kOpWords = int(kOpFrag / 2);       // 32-bit words per op (6 KiB)
-constexpr int kRingWords = kOpWords / kThreads;  // per thread per op
-static_assert(kOpWords % kThreads == 0, "fragment ring split");
+// One op record in 32-bit words: the fragments (1536 words); the member
+// tables -- LDS byte offsets, relative to a group's slot, of the members a
+// lane reads (8) and writes (8), for lane half 0 then 1 (32 words); the
+// group table -- LDS byte offset of lane j's group within a batch, j < 32
+// (32 words); padded to 8 KiB.
+constexpr int kFragWords = int(kOpFrag / 2);
+constexpr int kTableWords = 32;
+constexpr int kLaneWords = 32;
+constexpr int kOpWords = 2048;
+static_assert(kFragWords + kTableWords + kLaneWords <= kOpWords, "op record");
+constexpr int kRingQuads = kOpWords / 4 / kThreads;  // uint4 per thread per op
+static_assert(kOpWords % (4 * kThreads) == 0, "fragment ring split");
 constexpr size_t kBlockLds = kTileBytes + 2 * kOpWords * sizeof(uint32_t);
 
-// One 4-qubit op over the LDS tile: wave `wave` of `nwaves` takes batches
-// of 32 groups; `f` holds the op's fragments (hi, mid, lo per K block).
-__device__ __forceinline__ void ApplyTileOp(float2* lds, const uint8_t* tp,
-                                            const v8bf (&f)[6], unsigned nbatch,
-                                            int wave, int nwaves) {
-  const int l = threadIdx.x & 63;
-  const int j = l & 31, h = l >> 5;
-  const unsigned t0 = tp[0], t1 = tp[1], t2 = tp[2], t3 = tp[3];
-  auto toff = [&](unsigned m) {
-    return ((m & 1u) << t0) | (((m >> 1) & 1u) << t1) |
-           (((m >> 2) & 1u) << t2) | (((m >> 3) & 1u) << t3);
-  };
-  // TileSwz is linear over GF(2) and a member's offset has no bit in common
-  // with its group base, so slot(base + off) = TileSwz(base) ^ TileSwz(off):
-  // per member one XOR with a per-op constant (the lane half's part of the
-  // member index folded in once).
-  const unsigned sh = TileSwz(toff(4u * unsigned(h)));
-  unsigned rd[8], wr[8];
+// The tile at LDS byte offset `b`: the kernel has no static LDS, so its
+// dynamic block starts at offset 0 (checked on the host before launch), and
+// an address is the offset itself, with no base to add.
+typedef __attribute__((address_space(3))) v2f LdsV2f;
+__device__ __forceinline__ LdsV2f* LdsAt(unsigned b) {
+  return reinterpret_cast<LdsV2f*>(size_t{b});
+}
+
+// One 4-qubit op over the LDS tile.  A group (the 16 amplitudes the op
+// mixes) is fixed by its bits at the tile's non-target positions; the host
+// splits those positions into 5 lane positions (chosen so that the 32
+// groups of a batch fall into distinct LDS banks) and tb - 9 batch
+// positions `bp`.  Wave `wave` (wave-uniform) of `nwaves` takes batches
+// bt = wave, wave + nwaves, ...; lanes j and j + 32 serve the batch's j-th
+// group.  `f` holds the op's fragments (hi, mid, lo per K block); rd / wr
+// are this lane's member offsets with its group's offset already folded in
+// (LDS bytes).  TileSwz is linear over GF(2) and the lane part, batch part
+// and member offsets occupy disjoint bits, so a member's slot is the XOR of
+// the three parts' swizzled slots: the batch part is scalar work and an
+// address is one XOR (the tile sits at LDS offset 0).
+__device__ __forceinline__ unsigned BatchSlot(unsigned bt, unsigned b0,
+                                              unsigned b1, unsigned b2,
+                                              unsigned b3) {
+  const unsigned pb = ((bt & 1u) << b0) | (((bt >> 1) & 1u) << b1) |
+                      (((bt >> 2) & 1u) << b2) | (((bt >> 3) & 1u) << b3);
+  return TileSwz(pb) * unsigned(sizeof(float2));
+}
+
+// Lane half h's 8 members of its group as bf16 (hi, lo) pairs.
+__device__ __forceinline__ void SplitBf16(const v2f (&v)[8], v8bf (&bh)[2],
+                                          v8bf (&bl)[2]) {
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
-    rd[q] = sh ^ TileSwz(toff(8u * unsigned(q >> 2) + unsigned(q & 3)));
+    const v2bf hi = __builtin_convertvector(v[q], v2bf);
+    const v2f r = v[q] - __builtin_convertvector(hi, v2f);
+    const v2bf lo = __builtin_convertvector(r, v2bf);
+    bh[q >> 2][2 * (q & 3)] = hi[0];
+    bh[q >> 2][2 * (q & 3) + 1] = hi[1];
+    bl[q >> 2][2 * (q & 3)] = lo[0];
+    bl[q >> 2][2 * (q & 3) + 1] = lo[1];
   }
-  const unsigned shw = TileSwz(toff(2u * unsigned(h)));
+}
+
+// The op on one batch: Al Bh + Am Bh + Ah Bl + Ah Bh per K block, one
+// accumulator chain.
+__device__ __forceinline__ v16f MfmaChain(const v8bf (&f)[6],
+                                          const v8bf (&bh)[2],
+                                          const v8bf (&bl)[2]) {
+  v16f c = v16f{};
+  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[2], bh[0], c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[1], bh[0], c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[0], bl[0], c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[0], bh[0], c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[5], bh[1], c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[4], bh[1], c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[3], bl[1], c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[3], bh[1], c, 0, 0, 0);
+  return c;
+}
+
+// Lane l holds (re, im) of members 4 s + 2 h + {0, 1}.
+__device__ __forceinline__ void WriteBatch(unsigned sp, const uint32_t (&wr)[8],
+                                           const v16f& c) {
 #pragma unroll
   for (int q = 0; q < 4; ++q)
 #pragma unroll
-    for (int u = 0; u < 2; ++u) wr[2 * q + u] = shw ^ TileSwz(toff(4u * q + u));
-  for (unsigned bt = wave; bt < nbatch; bt += nwaves) {
-    unsigned p0 = bt * 32 + j;
-    p0 = ((p0 >> t0) << (t0 + 1)) | (p0 & ((1u << t0) - 1));
-    p0 = ((p0 >> t1) << (t1 + 1)) | (p0 & ((1u << t1) - 1));
-    p0 = ((p0 >> t2) << (t2 + 1)) | (p0 & ((1u << t2) - 1));
-    p0 = ((p0 >> t3) << (t3 + 1)) | (p0 & ((1u << t3) - 1));
-    const unsigned sp = TileSwz(p0);
-    // Lane half h supplies members 8 kb + 4 h + 0..3 of group j, split into
-    // bf16 (hi, lo) pairs two values at a time.
-    v8bf bh[2], bl[2];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const float2 v = lds[sp ^ rd[q]];
-      const v2f x = {v.x, v.y};
-      const v2bf hi = __builtin_convertvector(x, v2bf);
-      const v2f r = x - __builtin_convertvector(hi, v2f);
-      const v2bf lo = __builtin_convertvector(r, v2bf);
-      bh[q >> 2][2 * (q & 3)] = hi[0];
-      bh[q >> 2][2 * (q & 3) + 1] = hi[1];
-      bl[q >> 2][2 * (q & 3)] = lo[0];
-      bl[q >> 2][2 * (q & 3) + 1] = lo[1];
+    for (int u = 0; u < 2; ++u) {
+      *LdsAt(sp ^ wr[2 * q + u]) = v2f{c[4 * q + 2 * u], c[4 * q + 2 * u + 1]};
     }
-    // One accumulator: the vector ALU, not MFMA latency, bounds this loop
-    // (other waves cover the dependent chain), so no extra adds.
-    v16f c = v16f{};
-    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[2], bh[0], c, 0, 0, 0);
-    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[1], bh[0], c, 0, 0, 0);
-    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[0], bl[0], c, 0, 0, 0);
-    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[0], bh[0], c, 0, 0, 0);
-    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[5], bh[1], c, 0, 0, 0);
-    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[4], bh[1], c, 0, 0, 0);
-    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[3], bl[1], c, 0, 0, 0);
-    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[3], bh[1], c, 0, 0, 0);
-    // Lane l holds (re, im) of members 4 s + 2 h + {0, 1}.
+}
+
+__device__ __forceinline__ void ApplyTileOp(const uint8_t* bp,
+                                            const v8bf (&f)[6],
+                                            const uint32_t (&rd)[8],
+                                            const uint32_t (&wr)[8],
+                                            unsigned nbatch, unsigned wave,
+                                            unsigned nwaves) {
+  const unsigned b0 = bp[0], b1 = bp[1], b2 = bp[2], b3 = bp[3];
+  // Two batches at a time (their groups are disjoint): batch 1's reads and
+  // split overlap batch 0's MFMA chain.
+  unsigned bt = wave;
+  for (; bt + nwaves < nbatch; bt += 2 * nwaves) {
+    const unsigned sp0 = BatchSlot(bt, b0, b1, b2, b3);
+    const unsigned sp1 = BatchSlot(bt + nwaves, b0, b1, b2, b3);
+    v2f v0[8], v1[8];
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
+    for (int q = 0; q < 8; ++q) v0[q] = *LdsAt(sp0 ^ rd[q]);
+    v8bf bh0[2], bl0[2], bh1[2], bl1[2];
+    SplitBf16(v0, bh0, bl0);
 #pragma unroll
-      for (int u = 0; u < 2; ++u) {
-        lds[sp ^ wr[2 * q + u]] = float2{c[4 * q + 2 * u], c[4 * q + 2 * u + 1]};
-      }
+    for (int q = 0; q < 8; ++q) v1[q] = *LdsAt(sp1 ^ rd[q]);
+    const v16f c0 = MfmaChain(f, bh0, bl0);
+    SplitBf16(v1, bh1, bl1);
+    const v16f c1 = MfmaChain(f, bh1, bl1);
+    WriteBatch(sp0, wr, c0);
+    WriteBatch(sp1, wr, c1);
+  }
+  if (bt < nbatch) {
+    const unsigned sp0 = BatchSlot(bt, b0, b1, b2, b3);
+    v2f v0[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v0[q] = *LdsAt(sp0 ^ rd[q]);
+    v8bf bh0[2], bl0[2];
+    SplitBf16(v0, bh0, bl0);
+    WriteBatch(sp0, wr, MfmaChain(f, bh0, bl0));
   }
 }
 
@@ -132,19 +179,29 @@ __device__ __forceinline__ uint64_t DepositHigh(unsigned x, const BlockGeom& g) 
 }
 
 // Tile staging with addresses split into a per-thread part and a per-element
-// part.  Element i of thread t is tile-local amplitude a = t + NT i.  The
-// host keeps the low run within the thread bits (Lc <= log2 NT = 9), so a's
-// low bits come from t alone and its high index is
-// (t >> Lc) | (i << (9 - Lc)) with disjoint bits: its offset is
-// D(t >> Lc) + D(i << (9 - Lc)), one add per element instead of a deposit
-// loop.  The element parts are wave-uniform and kept in units of 2^Lc
-// amplitudes (32 bits: no tile qubit is 32 above the low run).
+// part.  Element i of thread t is tile-local amplitude a = t + NT i (valid
+// for i < 2^tb / NT, a wave-uniform test).  The host keeps the low run
+// within the thread bits (Lc <= log2 NT), so a's low bits come from t alone
+// and its high index is (t >> Lc) | (i << (log2 NT - Lc)) with disjoint
+// bits: its offset is D(t >> Lc) + D(i << (log2 NT - Lc)), one add per
+// element instead of a deposit loop.  The element parts are wave-uniform
+// and kept in units of 2^Lc amplitudes (32 bits: no tile qubit is 32 above
+// the low run).
 template <int PER, int NT>
 struct TileIo {
-  static_assert(NT == 512, "thread bits");
-  static constexpr unsigned kLogNT = 9;
+  static_assert(NT == 256 || NT == 512, "thread bits");
+  static constexpr unsigned kLogNT = NT == 256 ? 8 : 9;
   uint64_t tofs;       // this thread's part
   uint32_t eofs[PER];  // the element parts >> Lc
+
+  // LDS slot of element i: TileSwz is linear and changes only the low four
+  // bits, so TileSwz(t + NT i) = (TileSwz(t) ^ (TileSwz(NT i) & 15)) + NT i
+  // with TileSwz(t) < NT: the NT i part is an instruction offset and the
+  // rest takes one XOR with a constant.
+  __device__ __forceinline__ unsigned Slot(int i) const {
+    return (TileSwz(threadIdx.x) ^ (TileSwz(NT * unsigned(i)) & 15u)) +
+           NT * unsigned(i);
+  }
 
   __device__ __forceinline__ void Init(const BlockGeom& g) {
     const unsigned t = threadIdx.x;
@@ -155,100 +212,199 @@ struct TileIo {
     }
   }
 
-  template <int CH>
-  __device__ __forceinline__ void Load(const float2* __restrict__ s,
-                                       uint64_t base, const BlockGeom& g,
-                                       float2* lds) const {
-    const unsigned T = 1u << g.tb;
+  // Issues the loads of the tile at `base` into v (this thread's elements).
+  template <bool FULL>
+  __device__ __forceinline__ void Fetch(const float2* __restrict__ s,
+                                        uint64_t base, const BlockGeom& g,
+                                        float2 (&v)[PER]) const {
+    const unsigned nper = (1u << g.tb) >> kLogNT;  // elements per thread
     const float2* p = s + base + tofs;
 #pragma unroll
-    for (int i0 = 0; i0 < PER; i0 += CH) {
-      float2 v[CH];
-#pragma unroll
-      for (int i = 0; i < CH; ++i) {
-        const unsigned a = threadIdx.x + NT * (i0 + i);
-        if (a < T) v[i] = p[uint64_t(eofs[i0 + i]) << g.Lc];
-      }
-#pragma unroll
-      for (int i = 0; i < CH; ++i) {
-        const unsigned a = threadIdx.x + NT * (i0 + i);
-        if (a < T) lds[TileSwz(a)] = v[i];
-      }
+    for (int i = 0; i < PER; ++i) {
+      if (FULL || unsigned(i) < nper) v[i] = p[uint64_t(eofs[i]) << g.Lc];
     }
   }
 
-  template <int CH>
+  // Writes fetched elements to their LDS slots.
+  template <bool FULL>
+  __device__ __forceinline__ void Put(const BlockGeom& g,
+                                      const float2 (&v)[PER],
+                                      float2* lds) const {
+    const unsigned nper = (1u << g.tb) >> kLogNT;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      if (FULL || unsigned(i) < nper) lds[Slot(i)] = v[i];
+    }
+  }
+
+  template <int CH, bool FULL>
   __device__ __forceinline__ void Store(float2* __restrict__ s, uint64_t base,
                                         const BlockGeom& g,
                                         const float2* lds) const {
-    const unsigned T = 1u << g.tb;
+    const unsigned nper = (1u << g.tb) >> kLogNT;
     float2* p = s + base + tofs;
 #pragma unroll
     for (int i0 = 0; i0 < PER; i0 += CH) {
       float2 v[CH];
 #pragma unroll
       for (int i = 0; i < CH; ++i) {
-        const unsigned a = threadIdx.x + NT * (i0 + i);
-        if (a < T) v[i] = lds[TileSwz(a)];
+        if (FULL || unsigned(i0 + i) < nper) v[i] = lds[Slot(i0 + i)];
       }
 #pragma unroll
       for (int i = 0; i < CH; ++i) {
-        const unsigned a = threadIdx.x + NT * (i0 + i);
-        if (a < T) p[uint64_t(eofs[i0 + i]) << g.Lc] = v[i];
+        if (FULL || unsigned(i0 + i) < nper) p[uint64_t(eofs[i0 + i]) << g.Lc] = v[i];
       }
     }
   }
 };
 
-// Two 512-thread workgroups per CU, each with a 64 KiB tile and a two-slot
-// ring of op fragments in LDS: while op i runs from slot i % 2, each thread
-// holds its 12 bytes of op i + 1's fragments in flight from global memory
+// Two 256-thread workgroups per CU, each with a 64 KiB tile and a two-slot
+// ring of op records in LDS: while op i runs from slot i % 2, each thread
+// holds its 32 bytes of op i + 1's record in flight from global memory
 // (L2) and writes them to the other slot before the op's barrier, so the
-// fragment latency hides behind the op's MFMA work.  The program repeats for
-// every tile, so the ring wraps from the last op to the first.
+// record latency hides behind the op's MFMA work.  The program repeats for
+// every tile, so the ring wraps from the last op to the first.  One wave
+// per SIMD per workgroup leaves each wave 256 registers: enough to keep two
+// batches of an op in flight and the next tile's amplitudes (64 registers)
+// in flight from HBM during the last op and the store.  Vector memory
+// counters complete in order, so that prefetch is issued after the last
+// op's record fetch and never delays a record.
+// FULL: the tile holds 2^13 amplitudes, every thread's 32 elements.
+template <bool FULL>
 __global__ void __launch_bounds__(kThreads)
-__attribute__((amdgpu_waves_per_eu(4)))
+__attribute__((amdgpu_waves_per_eu(2)))
 GateBlockKernel(float2* __restrict__ s, BlockGeom g, BlockProgram p,
                 const uint16_t* __restrict__ frag) {
   extern __shared__ char lds_raw[];
   float2* lds = reinterpret_cast<float2*>(lds_raw);
   uint32_t* ring = reinterpret_cast<uint32_t*>(lds_raw + kTileBytes);
-  const int l = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int l = threadIdx.x & 63;
+  const unsigned wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const unsigned nbatch = (1u << g.tb) >> 9;  // 32 groups of 16 per batch
   constexpr int PER = (1 << kBlockMaxBits) / kThreads;
-  const uint32_t* gfrag = reinterpret_cast<const uint32_t*>(frag);
-  // Op 0's fragments into slot 0.
+  const uint4* gfrag = reinterpret_cast<const uint4*>(frag);
+  uint4* ring4 = reinterpret_cast<uint4*>(ring);
+  // Op 0's record into slot 0.
 #pragma unroll
-  for (int w = 0; w < kRingWords; ++w) {
-    ring[threadIdx.x + kThreads * w] = gfrag[threadIdx.x + kThreads * w];
+  for (int w = 0; w < kRingQuads; ++w) {
+    ring4[threadIdx.x + kThreads * w] = gfrag[threadIdx.x + kThreads * w];
   }
   TileIo<PER, kThreads> io;
   io.Init(g);
   unsigned k = 0;  // ops run so far: the current op's slot is k % 2
-  for (uint64_t tile = blockIdx.x; tile < g.ntiles; tile += gridDim.x) {
-    const uint64_t base = InsertZeros(tile << g.Lc, g.outer, g.nouter);
-    io.Load<16>(s, base, g, lds);
-    __syncthreads();
-    for (unsigned op = 0; op < p.nops; ++op, ++k) {
-      const unsigned nxt = op + 1 == p.nops ? 0 : op + 1;
-      const uint32_t* gn = gfrag + size_t(nxt) * kOpWords;
-      uint32_t pre[kRingWords];
-#pragma unroll
-      for (int w = 0; w < kRingWords; ++w) pre[w] = gn[threadIdx.x + kThreads * w];
-      const v8bf* fp =
-          reinterpret_cast<const v8bf*>(ring + (k & 1u) * kOpWords);
+  // Op `op` on the tile in LDS; `extra` runs once op nxt's record loads are
+  // issued.
+  auto run_op = [&](unsigned op, unsigned nxt, auto&& extra) {
+      const uint4* gn = gfrag + size_t(nxt) * (kOpWords / 4);
+      static_assert(kRingQuads == 2, "record split");
+      const uint4 pre0 = gn[threadIdx.x];
+      const uint4 pre1 = gn[threadIdx.x + kThreads];
+      extra();
+      const uint32_t* cur = ring + (k & 1u) * kOpWords;
+      const v8bf* fp = reinterpret_cast<const v8bf*>(cur);
       v8bf f[6];
 #pragma unroll
       for (int r = 0; r < 6; ++r) f[r] = fp[r * 64 + l];
-      ApplyTileOp(lds, p.tpos[op], f, nbatch, wave, kWaves);
-      uint32_t* slot = ring + ((k + 1) & 1u) * kOpWords;
+      const uint32_t* tab = cur + kFragWords + 16 * (l >> 5);
+      const uint32_t sj = cur[kFragWords + kTableWords + (l & 31)];
+      uint32_t rd[8], wr[8];
 #pragma unroll
-      for (int w = 0; w < kRingWords; ++w) slot[threadIdx.x + kThreads * w] = pre[w];
+      for (int q = 0; q < 8; ++q) {
+        rd[q] = tab[q] ^ sj;
+        wr[q] = tab[8 + q] ^ sj;
+      }
+      ApplyTileOp(p.bpos[op], f, rd, wr, nbatch, wave, kWaves);
+      uint4* slot = ring4 + ((k + 1) & 1u) * (kOpWords / 4);
+      slot[threadIdx.x] = pre0;
+      slot[threadIdx.x + kThreads] = pre1;
       __syncthreads();
-    }
-    io.Store<16>(s, base, g, lds);
+      ++k;
+  };
+  float2 buf[PER];
+  uint64_t tile = blockIdx.x;
+  if (tile < g.ntiles) {
+    io.template Fetch<FULL>(s, InsertZeros(tile << g.Lc, g.outer, g.nouter), g,
+                            buf);
+    io.template Put<FULL>(g, buf, lds);
+  }
+  __syncthreads();
+  // Each iteration leaves the next tile in LDS, so the waits on the
+  // prefetch are counted within one iteration.
+  for (; tile < g.ntiles; tile += gridDim.x) {
+    const uint64_t base = InsertZeros(tile << g.Lc, g.outer, g.nouter);
+    // The next tile of this workgroup, or this one again past the end (an
+    // in-bounds read that is never used), so the prefetch is unconditional.
+    const uint64_t next = tile + gridDim.x < g.ntiles ? tile + gridDim.x : tile;
+    const uint64_t nbase = InsertZeros(next << g.Lc, g.outer, g.nouter);
+    for (unsigned op = 0; op + 1 < p.nops; ++op) run_op(op, op + 1, [] {});
+    run_op(p.nops - 1, 0,
+           [&] { io.template Fetch<FULL>(s, nbase, g, buf); });
+    io.template Store<16, FULL>(s, base, g, lds);
+    __syncthreads();
+    io.template Put<FULL>(g, buf, lds);
     __syncthreads();
   }
+}
+
+// TileSwz on the host (the slot of tile-local amplitude a).
+unsigned HostTileSwz(unsigned a) { return a ^ (((a >> 4) ^ (a >> 8)) & 15u); }
+
+// x's low k bits placed at positions pos[0..k).
+unsigned Deposit(unsigned x, const unsigned* pos, unsigned k) {
+  unsigned o = 0;
+  for (unsigned b = 0; b < k; ++b) o |= ((x >> b) & 1u) << pos[b];
+  return o;
+}
+
+// Splits the tb - 4 non-target positions of an op (targets `pad`) into 5
+// lane positions and tb - 9 batch positions (ascending; unused entries 0).
+// The 32 groups of a batch are read in quarter-waves of 16 lanes and half
+// waves of 32; the choice minimises first the largest number of lanes of a
+// quarter-wave whose group slots share a bank (slot mod 16: 8-byte slots
+// over 32 four-byte banks), then the same over the half-wave (slot mod 32),
+// so that the op's LDS reads and writes are conflict-free where the
+// targets allow it.
+void ChooseLanePositions(const unsigned* pad, unsigned tb, unsigned* lane,
+                         unsigned* batch) {
+  unsigned np = 0, nonk[kBlockMaxBits];
+  for (unsigned q = 0; q < tb; ++q) {
+    if (q != pad[0] && q != pad[1] && q != pad[2] && q != pad[3]) nonk[np++] = q;
+  }
+  auto worst = [](const unsigned* sl, unsigned n, unsigned mod) {
+    unsigned cnt[32] = {}, w = 0;
+    for (unsigned j = 0; j < n; ++j) w = std::max(w, ++cnt[sl[j] % mod]);
+    return w;
+  };
+  unsigned best = ~0u;
+  unsigned cand[5];
+  // Ordered choice: a 4-subset for lane bits 0..3 (kept ascending), then
+  // one more position for lane bit 4.
+  for (unsigned a = 0; a < np && best != 0; ++a)
+    for (unsigned b = a + 1; b < np && best != 0; ++b)
+      for (unsigned c = b + 1; c < np && best != 0; ++c)
+        for (unsigned d = c + 1; d < np && best != 0; ++d)
+          for (unsigned e = 0; e < np && best != 0; ++e) {
+            if (e == a || e == b || e == c || e == d) continue;
+            cand[0] = nonk[a];
+            cand[1] = nonk[b];
+            cand[2] = nonk[c];
+            cand[3] = nonk[d];
+            cand[4] = nonk[e];
+            unsigned sl[32];
+            for (unsigned j = 0; j < 32; ++j) sl[j] = HostTileSwz(Deposit(j, cand, 5));
+            const unsigned score =
+                ((std::max(worst(sl, 16, 16), worst(sl + 16, 16, 16)) - 1) << 8) |
+                (worst(sl, 32, 32) - 1);
+            if (score < best) {
+              best = score;
+              std::copy(cand, cand + 5, lane);
+            }
+          }
+  unsigned nb = 0;
+  for (unsigned q = 0; q < np; ++q) {
+    if (std::find(lane, lane + 5, nonk[q]) == lane + 5) batch[nb++] = nonk[q];
+  }
+  for (; nb < 4; ++nb) batch[nb] = 0;
 }
 
 // U (2^k x 2^k over positions `pos`, ascending) embedded into the 16 x 16
@@ -315,12 +471,12 @@ int PrepareGateBlock(unsigned n, const unsigned* tile, unsigned nt,
              "and add at most %u higher ones", 2u, kBlockMaxHigh);
     return QSIM_HIP_ERR_ARG;
   }
-  // The kernel's staging keeps the contiguous run within its 9 thread bits;
+  // The kernel's staging keeps the contiguous run within its 8 thread bits;
   // further low qubits are handled as tile qubits above the run (the same
   // tile-local positions and addresses).
-  if (g.Lc > 9) {
-    g.Lc = 9;
-    g.h = nt - 9;
+  if (g.Lc > 8) {
+    g.Lc = 8;
+    g.h = nt - 8;
   }
   g.nouter = g.h;
   for (unsigned b = 0; b < g.h; ++b) {
@@ -339,7 +495,8 @@ int PrepareGateBlock(unsigned n, const unsigned* tile, unsigned nt,
 
   p = BlockProgram{};
   p.nops = nops;
-  fb.assign(std::max<size_t>(nops, 1) * kOpFrag, 0);
+  constexpr size_t kOpElems = 2 * size_t(kOpWords);  // uint16 per op record
+  fb.assign(std::max<size_t>(nops, 1) * kOpElems, 0);
   std::vector<double> u16(2 * 16 * 16);
   for (unsigned i = 0; i < nops; ++i) {
     const unsigned k = ks[i];
@@ -373,7 +530,34 @@ int PrepareGateBlock(unsigned n, const unsigned* tile, unsigned nt,
       p.tpos[i][b] = uint8_t(merged[b]);
     }
     PadTo4(m64 + moff[i], k, pos, pad, u16.data());
-    BuildFragBf16(4, u16.data(), fb.data() + i * kOpFrag);
+    uint16_t* rec = fb.data() + i * kOpElems;
+    BuildFragBf16(4, u16.data(), rec);
+    // Lane tables: member m of a group sits at tile offset toff(m) (bit b of
+    // m at position pad[b]); lane half h reads members 4h + 8(q/4) + q%4 and
+    // writes members 2h + 4q + u (the operand maps of the MFMA product).
+    auto toff = [&](unsigned m) {
+      unsigned o = 0;
+      for (int b = 0; b < 4; ++b) o |= ((m >> b) & 1u) << pad[b];
+      return o;
+    };
+    uint32_t tab[kTableWords + kLaneWords];
+    for (unsigned h = 0; h < 2; ++h) {
+      for (unsigned q = 0; q < 8; ++q) {
+        tab[16 * h + q] = uint32_t(sizeof(float2)) *
+                          HostTileSwz(toff(4 * h + 8 * (q >> 2) + (q & 3)));
+        tab[16 * h + 8 + q] = uint32_t(sizeof(float2)) *
+                              HostTileSwz(toff(2 * h + 4 * (q >> 1) + (q & 1)));
+      }
+    }
+    // Group table: lane j's group has j's bits at the lane positions.
+    unsigned lane[5], batch[4];
+    ChooseLanePositions(pad, nt, lane, batch);
+    for (unsigned j = 0; j < 32; ++j) {
+      tab[kTableWords + j] =
+          uint32_t(sizeof(float2)) * HostTileSwz(Deposit(j, lane, 5));
+    }
+    for (int b = 0; b < 4; ++b) p.bpos[i][b] = uint8_t(batch[b]);
+    std::memcpy(rec + 2 * size_t(kFragWords), tab, sizeof(tab));
   }
 
   return QSIM_HIP_OK;
@@ -381,21 +565,41 @@ int PrepareGateBlock(unsigned n, const unsigned* tile, unsigned nt,
 
 int RunGateBlock(float2* s, const BlockGeom& g, const BlockProgram& p,
                  const uint16_t* dfrag) {
-  // More than 64 KiB of dynamic LDS must be allowed once per process.
+  // More than 64 KiB of dynamic LDS must be allowed once per process, and
+  // the kernels address LDS from offset 0, which holds only with no static
+  // LDS ahead of the dynamic block.
   static std::once_flag once;
   static hipError_t attr = hipSuccess;
+  static size_t static_lds = 0;
   std::call_once(once, [] {
-    attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&GateBlockKernel),
-                               hipFuncAttributeMaxDynamicSharedMemorySize,
-                               int(kBlockLds));
+    const void* fns[2] = {reinterpret_cast<const void*>(&GateBlockKernel<true>),
+                          reinterpret_cast<const void*>(&GateBlockKernel<false>)};
+    for (const void* fn : fns) {
+      hipFuncAttributes fa{};
+      if (attr == hipSuccess) attr = hipFuncGetAttributes(&fa, fn);
+      static_lds = std::max(static_lds, fa.sharedSizeBytes);
+      if (attr == hipSuccess) {
+        attr = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   int(kBlockLds));
+      }
+    }
   });
   QHIP_CHECK(attr);
-  // Two workgroups per CU fit in LDS (2 x 76 KiB); a few tiles each.
+  if (static_lds != 0) {
+    SetError("blocked-pass kernel has %zu bytes of static LDS", static_lds);
+    return QSIM_HIP_ERR_RUNTIME;
+  }
+  // Two workgroups per CU fit in LDS (2 x 80 KiB); a few tiles each.
   const uint64_t blocks =
       std::min<uint64_t>(g.ntiles, uint64_t(NumCUs()) * 2 * 8);
   PassTimerBegin();
-  hipLaunchKernelGGL(GateBlockKernel, dim3(unsigned(blocks)), dim3(kThreads),
-                     kBlockLds, Stream(), s, g, p, dfrag);
+  if (g.tb == kBlockMaxBits) {
+    hipLaunchKernelGGL(GateBlockKernel<true>, dim3(unsigned(blocks)),
+                       dim3(kThreads), kBlockLds, Stream(), s, g, p, dfrag);
+  } else {
+    hipLaunchKernelGGL(GateBlockKernel<false>, dim3(unsigned(blocks)),
+                       dim3(kThreads), kBlockLds, Stream(), s, g, p, dfrag);
+  }
   PassTimerEnd();
   QHIP_LAUNCHED("GateBlockKernel");
   return QSIM_HIP_OK;

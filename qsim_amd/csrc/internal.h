@@ -127,6 +127,9 @@ struct BlockGeom {
 struct BlockProgram {
   unsigned nops;
   uint8_t tpos[kBlockMaxOps][4];  // tile-local targets of each op, ascending
+  // Tile-local positions that carry the batch index of each op's groups
+  // (tb - 9 of them; unused entries are 0 and only ever see zero bits).
+  uint8_t bpos[kBlockMaxOps][4];
 };
 
 __host__ __device__ inline uint64_t InsertZeros(uint64_t i, const unsigned* pos,
