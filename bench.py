@@ -68,7 +68,7 @@ PATHS = {
 # HBM bytes per amplitude per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE
 # runs, scaled by what the counters read for a copy of known size (reads x 2,
 # writes x 1); profiles/gate_passes_32q_blocked_vs_fused.md.
-MEASURED_HBM_BYTES_PER_AMP = {"GateBlockKernel": 16.0}
+MEASURED_HBM_BYTES_PER_AMP = {"GateBlockKernel": 16.06}
 
 # One 32-group batch of a 4-qubit op in a blocked pass: 8 MFMAs of
 # 32 x 32 x 16 (2 K blocks x 4 split products).
