@@ -87,6 +87,18 @@ class StateSpaceHIP
     return std::complex<fp_type>(fp_type(re), fp_type(im));
   }
 
+  // Many amplitudes in one gather pass (no reference counterpart: it
+  // replaces loops of GetAmpl, e.g. apps/qsim_amplitudes.cc:138-153).
+  static std::vector<std::complex<fp_type>> GetAmpls(
+      const State& state, const std::vector<uint64_t>& indices) {
+    std::vector<std::complex<fp_type>> out(indices.size());
+    hip_detail::Check(
+        qsim_hip_get_ampls(kPrec, state.get(), state.num_qubits(),
+                           indices.data(), indices.size(), out.data()),
+        "GetAmpls");
+    return out;
+  }
+
   static void SetAmpl(State& state, uint64_t i,
                       const std::complex<fp_type>& ampl) {
     SetAmpl(state, i, std::real(ampl), std::imag(ampl));

@@ -119,6 +119,16 @@ int qsim_hip_get_ampl(int prec, const void* state, unsigned num_qubits,
 int qsim_hip_set_ampl(int prec, void* state, unsigned num_qubits, uint64_t i,
                       double re, double im);
 
+/* out[j] = state[indices[j]] for j < count: one gather pass and one copy
+ * instead of a GetAmpl per bitstring.  `out` is host memory of `count`
+ * complex values in the state's precision (float or double pairs); every
+ * index must be < 2^num_qubits (QSIM_HIP_ERR_ARG otherwise, nothing read).
+ * Replaces the per-bitstring GetAmpl loops of apps/qsim_amplitudes.cc:138-153
+ * (WriteAmplitudes) and lib/hybrid.h:428-438 (the results collection);
+ * no single reference function corresponds. */
+int qsim_hip_get_ampls(int prec, const void* state, unsigned num_qubits,
+                       const uint64_t* indices, uint64_t count, void* out);
+
 /* state[i] = re + i im where (i & mask) == bits (or != bits if exclude)
  * (lib/statespace_cuda.h:168-187). */
 int qsim_hip_bulk_set_ampl(int prec, void* state, unsigned num_qubits,

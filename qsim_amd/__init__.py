@@ -24,6 +24,12 @@ LIB_PATH = os.path.join(HERE, "lib", "libqsim_hip.so")
 
 F32 = 0
 F64 = 1
+# Status codes of include/qsim_hip.h.
+OK = 0
+ERR_ARG = -1
+ERR_RUNTIME = -2
+ERR_OOM = -3
+ERR_UNSUPPORTED = -4
 METHOD_AUTO = 0
 METHOD_VECTOR = 1
 METHOD_MFMA_BF16X3 = 2
@@ -62,6 +68,7 @@ SIGNATURES = [
     ("qsim_hip_normal_to_internal_order", _i, [_i, _vp, _u]),
     ("qsim_hip_get_ampl", _i, [_i, _vp, _u, _u64, _f64p, _f64p]),
     ("qsim_hip_set_ampl", _i, [_i, _vp, _u, _u64, _d, _d]),
+    ("qsim_hip_get_ampls", _i, [_i, _vp, _u, _vp, _u64, _vp]),
     ("qsim_hip_bulk_set_ampl", _i, [_i, _vp, _u, _u64, _u64, _d, _d, _i]),
     ("qsim_hip_add", _i, [_i, _vp, _vp, _u]),
     ("qsim_hip_multiply", _i, [_i, _vp, _u, _d]),
@@ -305,6 +312,17 @@ class StateSpace:
         _check(lib().qsim_hip_get_ampl(s.prec, s.ptr, s.num_qubits, i,
                                        ctypes.byref(re), ctypes.byref(im)))
         return complex(re.value, im.value)
+
+    def get_ampls(self, s: State, indices) -> np.ndarray:
+        """Amplitudes of many basis states in one gather (complex64 or
+        complex128, as the state)."""
+        idx = np.ascontiguousarray(indices, dtype=np.uint64).reshape(-1)
+        out = np.empty(idx.size, dtype=np.complex64 if s.prec == F32
+                       else np.complex128)
+        _check(lib().qsim_hip_get_ampls(s.prec, s.ptr, s.num_qubits,
+                                        idx.ctypes.data, idx.size,
+                                        out.ctypes.data))
+        return out
 
     def set_ampl(self, s: State, i: int, val: complex):
         _check(lib().qsim_hip_set_ampl(s.prec, s.ptr, s.num_qubits, i,

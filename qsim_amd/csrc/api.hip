@@ -373,6 +373,26 @@ int qsim_hip_get_ampl(int prec, const void* s, unsigned n, uint64_t i,
   return QSIM_HIP_OK;
 }
 
+int qsim_hip_get_ampls(int prec, const void* s, unsigned n,
+                       const uint64_t* indices, uint64_t count, void* out) {
+  QHIP_TRY(CheckState(prec, s, n));
+  if (count == 0) return QSIM_HIP_OK;
+  if (indices == nullptr || out == nullptr) {
+    SetError("null amplitude index or output array");
+    return QSIM_HIP_ERR_ARG;
+  }
+  for (uint64_t j = 0; j < count; ++j) {
+    if (indices[j] >> n) {
+      SetError("amplitude index %llu out of range for %u qubits",
+               (unsigned long long)indices[j], n);
+      return QSIM_HIP_ERR_ARG;
+    }
+  }
+  return QHIP_DISPATCH(
+      prec, Gather<float>((const float2*)s, indices, count, (float2*)out),
+      Gather<double>((const double2*)s, indices, count, (double2*)out));
+}
+
 int qsim_hip_set_ampl(int prec, void* s, unsigned n, uint64_t i, double re,
                       double im) {
   QHIP_TRY(CheckState(prec, s, n));

@@ -175,6 +175,9 @@ template <typename FP>
 int Sample(const CT<FP>* s, unsigned n, const double* draws, uint64_t num,
            uint64_t* out);
 
+template <typename FP>
+int Gather(const CT<FP>* s, const uint64_t* idx, uint64_t count, CT<FP>* out);
+
 int Convert(void* dst, int dprec, const void* src, int sprec, unsigned n);
 
 // MFMA path for uncontrolled k>=5 gates on complex64 states.  `m64` is the

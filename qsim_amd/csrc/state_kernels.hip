@@ -463,6 +463,46 @@ int Sample(const C_t<FP>* s, unsigned n, const double* draws, uint64_t num,
   return QSIM_HIP_OK;
 }
 
+// out[j] = s[idx[j]]: the amplitudes of a batch of bitstrings in one pass.
+template <typename C>
+__global__ void __launch_bounds__(kBS)
+GatherKernel(const C* __restrict__ s, const uint64_t* __restrict__ idx,
+             uint64_t count, C* __restrict__ out) {
+  const uint64_t stride = uint64_t(gridDim.x) * kBS;
+  for (uint64_t j = uint64_t(blockIdx.x) * kBS + threadIdx.x; j < count;
+       j += stride) {
+    out[j] = s[idx[j]];
+  }
+}
+
+// Amplitudes of `count` indices (host arrays; indices checked by the caller)
+// in chunks that fit the scratch buffer: one upload, one gather launch and
+// one download per chunk.
+template <typename FP>
+int Gather(const C_t<FP>* s, const uint64_t* idx, uint64_t count,
+           C_t<FP>* out) {
+  constexpr uint64_t kChunk = uint64_t{1} << 20;
+  std::lock_guard<std::mutex> lock(ScratchLock());
+  const uint64_t cap = std::min(count, kChunk);
+  char* base = static_cast<char*>(
+      DeviceScratch(cap * (sizeof(uint64_t) + sizeof(C_t<FP>))));
+  if (base == nullptr) return QSIM_HIP_ERR_OOM;
+  uint64_t* d_idx = reinterpret_cast<uint64_t*>(base);
+  C_t<FP>* d_out = reinterpret_cast<C_t<FP>*>(d_idx + cap);
+  for (uint64_t j0 = 0; j0 < count; j0 += cap) {
+    const uint64_t m = std::min(cap, count - j0);
+    QHIP_CHECK(hipMemcpyAsync(d_idx, idx + j0, sizeof(uint64_t) * m,
+                              hipMemcpyHostToDevice, Stream()));
+    hipLaunchKernelGGL((GatherKernel<C_t<FP>>), dim3(GridFor(m)), dim3(kBS), 0,
+                       Stream(), s, d_idx, m, d_out);
+    QHIP_LAUNCHED("GatherKernel");
+    QHIP_CHECK(hipMemcpyAsync(out + j0, d_out, sizeof(C_t<FP>) * m,
+                              hipMemcpyDeviceToHost, Stream()));
+    QHIP_CHECK(hipStreamSynchronize(Stream()));
+  }
+  return QSIM_HIP_OK;
+}
+
 #define QHIP_INSTANTIATE(FP)                                                   \
   template int Fill<FP>(C_t<FP>*, unsigned, int);                              \
   template int BulkSet<FP>(C_t<FP>*, unsigned, uint64_t, uint64_t, double,     \
@@ -476,7 +516,8 @@ int Sample(const C_t<FP>* s, unsigned n, const double* draws, uint64_t num,
                                     double, uint64_t, uint64_t*);              \
   template int Collapse<FP>(C_t<FP>*, unsigned, uint64_t, uint64_t);           \
   template int Sample<FP>(const C_t<FP>*, unsigned, const double*, uint64_t,   \
-                          uint64_t*);
+                          uint64_t*);                                          \
+  template int Gather<FP>(const C_t<FP>*, const uint64_t*, uint64_t, C_t<FP>*);
 
 template <typename D, typename S>
 __global__ void __launch_bounds__(kBS)

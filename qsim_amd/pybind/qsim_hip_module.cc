@@ -1,0 +1,59 @@
+// qsim_hip_module.cc -- qsim's Python extension `qsim_hip` on the gfx950
+// backend.
+//
+// qsim's HIP extension (pybind_interface/hip/pybind_main_hip.cpp:18-51) is a
+// Factory over SimulatorCUDA<float> compiled into the backend-independent
+// bindings of pybind_interface/pybind_main.{h,cpp}.  This file supplies the
+// same Factory shape over this backend's SimulatorHIP / StateSpaceHIP and
+// compiles the same bindings (read from the qsim sources at build time), so
+// qsimcirq's `qsim_hip` import (`qsimcirq/__init__.py:34-39`) and every call
+// it makes (`qsim_simulate`, `qsim_simulate_fullstate`, samplers, expectation
+// values, qtrajectory and qsimh entry points) run here unchanged.
+
+#include "pybind_main.h"
+
+#include "fuser_mqubit.h"
+#include "gates_cirq.h"
+#include "io.h"
+#include "qsim_amd/factory_hip.h"
+#include "qtrajectory.h"
+#include "run_qsim.h"
+
+PYBIND11_MODULE(qsim_hip, m) { GPU_MODULE_BINDINGS }
+
+namespace qsim {
+
+struct Factory {
+  // The CUDA extension reads "gsst" / "gdb" (CUDA launch shape); the gfx950
+  // kernels size their launches themselves, so the options are accepted and
+  // not used.
+  explicit Factory(const py::dict&) {}
+
+  using Simulator = SimulatorHIP<float>;
+  using StateSpace = Simulator::StateSpace;
+
+  using Gate = Cirq::GateCirq<float>;
+  using Operation = qsim::Operation<float>;
+  using Runner = QSimRunner<IO, MultiQubitGateFuser<IO>, Factory>;
+  using RunnerParameter = Runner::Parameter;
+  using NoisyRunner = qsim::QuantumTrajectorySimulator<IO, Runner>;
+  using NoisyRunnerParameter = NoisyRunner::Parameter;
+
+  StateSpace CreateStateSpace() const { return StateSpace(); }
+  Simulator CreateSimulator() const { return Simulator(); }
+};
+
+// Denormal control is a host-CPU setting of the CPU simulators.
+inline void SetFlushToZeroAndDenormalsAreZeros() {}
+inline void ClearFlushToZeroAndDenormalsAreZeros() {}
+
+namespace detail {
+// The bindings name the backend's state deleter `detail::free` (the CUDA
+// backend's is lib/vectorspace_cuda.h:34-36); only the branch for states
+// that do not need a host copy calls it, which a device state never takes.
+inline void free(void* ptr) { hip_detail::free_device(ptr); }
+}  // namespace detail
+
+}  // namespace qsim
+
+#include "pybind_main.cpp"
