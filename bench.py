@@ -25,7 +25,8 @@ that path is checked against the CPU FP64 oracle by the tests), with the
 norm drift |<psi|psi> - 1| beside it.  `value` is the learned planner's
 pick when it meets 1 - F <= 1e-6 with a norm drift <= 1e-5, else the
 fastest path that does.  Beside the headline, the batched trajectory engine
-runs its own workload (1024 noisy trajectories of 18/20/22 qubits).
+runs its own workload (1024 noisy trajectories of 18/20/22 qubits), and the
+20-qubit grid circuit runs eagerly and as one captured graph.
 
 Prints one JSON line with the roofline of that path's dominant gate kernel
 (HIP events around every pass inside the timed region) and qsim's own
@@ -338,6 +339,51 @@ def run_trajectories(args):
     return out
 
 
+def run_small_circuit(args):
+    """The 20-qubit grid circuit (4x5, 14 cycles, seed 0, max_fused_size 2:
+    qsim's own CPU-runnable case) run eagerly, one pass at a time from the
+    host, and as one captured graph with every matrix resident on the
+    device.  At 8 MiB of state this workload is bound by per-pass host work
+    and launch latency, not by HBM; reported beside the headline."""
+    import qsim_amd
+    from qsim_amd import circuits, driver
+    n, cycles = 20, 14
+    c = driver.Circuit(circuits.grid_circuit(n, cycles, 0))
+    c.fuse(2)
+    ss = qsim_amd.StateSpace(np.float32)
+    st = ss.create(n)
+
+    def timed(fn, reps=20):
+        ss.set_state_zero(st)
+        fn()
+        qsim_amd.synchronize()
+        best = float("inf")
+        for _ in range(reps):
+            ss.set_state_zero(st)
+            qsim_amd.synchronize()
+            t0 = time.perf_counter()
+            fn()
+            qsim_amd.synchronize()
+            best = min(best, time.perf_counter() - t0)
+        return best
+
+    eager = timed(lambda: c.run(st))
+    want = st.to_numpy()
+    ss.set_state_zero(st)
+    g = c.capture(st)
+    graph = timed(g.launch)
+    same = bool(np.array_equal(st.to_numpy(), want))
+    g.free()
+    st.free()
+    log(f"20q grid circuit (k=2, {c.num_passes} passes): eager "
+        f"{eager * 1e3:.2f} ms, graph {graph * 1e3:.2f} ms, identical {same}")
+    return {"workload": "20q random grid circuit 4x5, 14 cycles, seed 0, "
+                        "complex64, max_fused_size 2",
+            "passes": c.num_passes, "eager_ms": eager * 1e3,
+            "graph_ms": graph * 1e3, "graph_speedup": eager / graph,
+            "graph_identical_to_eager": same}
+
+
 def trajectories_cpu(args, sizes):
     """qsim's QuantumTrajectorySimulator (oracle/_ref) on a sample of the
     batch: 4 trajectories per size, scaled to the batch's mix."""
@@ -387,6 +433,8 @@ def main():
     ap.add_argument("--trajectories", type=int, default=1024,
                     help="size of the trajectory batch reported beside the "
                          "headline (0: skip)")
+    ap.add_argument("--no-small", action="store_true",
+                    help="skip the 20-qubit eager vs captured-graph line")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -452,6 +500,14 @@ def main():
             log(f"trajectory batch failed: {e!r}")
             traj = {"error": repr(e)}
 
+    small = None
+    if not args.no_small:
+        try:
+            small = run_small_circuit(args)
+        except Exception as e:  # reported beside the headline, never hidden
+            log(f"small circuit failed: {e!r}")
+            small = {"error": repr(e)}
+
     plain = results.get("plain-fp32-vector")
     out = {
         "metric": METRIC,
@@ -487,6 +543,7 @@ def main():
         "roofline": b["roofline"],
         "cpu_baseline": cpu,
         "trajectory_batch": traj,
+        "small_circuit": small,
     }
     print(json.dumps(out), flush=True)
 

@@ -251,6 +251,33 @@ int qsim_hip_expectation_value(int prec, const void* state,
 int qsim_hip_set_gate_method(int method);
 int qsim_hip_get_gate_method(void);
 
+/* ---- captured pass sequences ---------------------------------------------
+ * A sequence of gate passes (any mix of qsim_hip_apply_gate,
+ * apply_controlled_gate, apply_gate_block, block_run and state fills; no
+ * call that returns data to the host) recorded once and replayed as one HIP
+ * graph, with every gate matrix resident on the device.  Use:
+ *   record_begin(&plan); <run the passes>; record_end(plan);
+ *   capture_begin(plan); <run the same passes again>; capture_end(plan);
+ *   launch(plan) ... launch(plan); free(plan);
+ * The recorded run executes normally; while recording, each staged matrix
+ * gets a device buffer of its own (kept by the plan) instead of a slot of
+ * the shared staging ring.  The captured run must issue the same passes in
+ * the same order (the same matrices): it reuses those buffers, stages
+ * nothing and only records kernel launches (on a stream of the plan's own;
+ * the state pointers are part of the graph).  capture_end fails with
+ * QSIM_HIP_ERR_ARG if the sequences differ.  launch enqueues the graph on
+ * the library stream (asynchronous, like a pass).  One recording or capture
+ * at a time per process.  No single reference counterpart: it replaces
+ * QSimRunner's per-gate host work (matrix upload and launch per fused gate,
+ * lib/simulator_cuda.h:338-343, lib/run_qsim.h:128-160) for repeated runs
+ * of a noiseless circuit. */
+int qsim_hip_graph_record_begin(void** plan);
+int qsim_hip_graph_record_end(void* plan);
+int qsim_hip_graph_capture_begin(void* plan);
+int qsim_hip_graph_capture_end(void* plan);
+int qsim_hip_graph_launch(void* plan);
+int qsim_hip_graph_free(void* plan);
+
 /* ---- timing ------------------------------------------------------------ */
 
 /* Records event `slot` (0..63) on the library stream. */

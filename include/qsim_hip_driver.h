@@ -138,6 +138,18 @@ int qsim_driver_run_passes(void* circuit, void* state, int prec,
 /* All passes. */
 int qsim_driver_run(void* circuit, void* state, int prec, unsigned seed);
 
+/* The whole circuit as one replayable HIP graph on `state` (complex64 or
+ * complex128): runs every pass once (recording, the state advances as in
+ * qsim_driver_run), then captures the same passes with their matrices and
+ * blocked-pass fragments resident on the device (qsim_hip_graph_*).  Each
+ * qsim_hip_graph_launch(*plan) then applies the circuit to `state` again in
+ * one launch.  Circuits with measurements are refused
+ * (QSIM_HIP_ERR_UNSUPPORTED: their passes depend on host draws).  The plan
+ * owns every matrix and blocked-pass fragment the graph reads, so it stays
+ * valid after the circuit is re-fused or freed; free it with
+ * qsim_hip_graph_free. */
+int qsim_driver_capture(void* circuit, void* state, int prec, void** plan);
+
 #ifdef __cplusplus
 }
 #endif

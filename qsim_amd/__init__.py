@@ -69,6 +69,12 @@ SIGNATURES = [
     ("qsim_hip_get_ampl", _i, [_i, _vp, _u, _u64, _f64p, _f64p]),
     ("qsim_hip_set_ampl", _i, [_i, _vp, _u, _u64, _d, _d]),
     ("qsim_hip_get_ampls", _i, [_i, _vp, _u, _vp, _u64, _vp]),
+    ("qsim_hip_graph_record_begin", _i, [ctypes.POINTER(ctypes.c_void_p)]),
+    ("qsim_hip_graph_record_end", _i, [_vp]),
+    ("qsim_hip_graph_capture_begin", _i, [_vp]),
+    ("qsim_hip_graph_capture_end", _i, [_vp]),
+    ("qsim_hip_graph_launch", _i, [_vp]),
+    ("qsim_hip_graph_free", _i, [_vp]),
     ("qsim_hip_bulk_set_ampl", _i, [_i, _vp, _u, _u64, _u64, _d, _d, _i]),
     ("qsim_hip_add", _i, [_i, _vp, _vp, _u]),
     ("qsim_hip_multiply", _i, [_i, _vp, _u, _d]),

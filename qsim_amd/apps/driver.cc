@@ -54,6 +54,8 @@ struct DriverCircuit {
   std::vector<Step> steps;
   // Gate method the partition runs with (-1: the library's current one).
   int method = -1;
+  // Set while qsim_driver_capture records and captures the passes.
+  bool capturing = false;
   // Prepared blocked passes (qsim_hip_block_prepare), one per block, made at
   // the first run.
   std::vector<void*> prepared;
@@ -118,6 +120,15 @@ void PassInfo(const FusedOp& op, unsigned* k, unsigned* qubits,
 // Blocked pass `index`: prepared (fragments on the device) at its first run,
 // one kernel launch per run after that.
 int RunBlock(DriverCircuit& d, size_t index, float* p, unsigned n) {
+  if (d.capturing) {
+    // A captured graph owns its data: the fragments go through the
+    // library's staging (the graph plan's buffers), not the circuit's
+    // prepared handles, which a later fuse may release.
+    const hip::BlockPass& b = d.blocks[index];
+    return qsim_hip_apply_gate_block(QSIM_HIP_F32, p, n, b.tile.data(),
+                                     b.tile.size(), b.ks.size(), b.ks.data(),
+                                     b.qs.data(), b.m.data(), QSIM_HIP_F64);
+  }
   if (d.prepared.size() != d.blocks.size()) {
     d.ReleasePrepared();
     d.prepared.assign(d.blocks.size(), nullptr);
@@ -541,6 +552,40 @@ int qsim_driver_plan_learned(void* c, void* model, unsigned max_candidates,
 int qsim_driver_run(void* c, void* state, int prec, unsigned seed) {
   return qsim_driver_run_passes(c, state, prec, 0, qsim_driver_num_passes(c),
                                 seed);
+}
+
+int qsim_driver_capture(void* c, void* state, int prec, void** plan) {
+  if (c == nullptr || state == nullptr || plan == nullptr) {
+    return QSIM_HIP_ERR_ARG;
+  }
+  *plan = nullptr;
+  const unsigned np = qsim_driver_num_passes(c);
+  for (unsigned i = 0; i < np; ++i) {
+    if (qsim_driver_pass_kind(c, i) == QSIM_DRIVER_PASS_MEASUREMENT) {
+      return QSIM_HIP_ERR_UNSUPPORTED;
+    }
+  }
+  auto* d = static_cast<DriverCircuit*>(c);
+  void* p = nullptr;
+  int rc = qsim_hip_graph_record_begin(&p);
+  if (rc != QSIM_HIP_OK) return rc;
+  d->capturing = true;
+  rc = qsim_driver_run(c, state, prec, 0);
+  int rc2 = qsim_hip_graph_record_end(p);
+  if (rc == QSIM_HIP_OK) rc = rc2;
+  if (rc == QSIM_HIP_OK) rc = qsim_hip_graph_capture_begin(p);
+  if (rc == QSIM_HIP_OK) {
+    rc = qsim_driver_run(c, state, prec, 0);
+    rc2 = qsim_hip_graph_capture_end(p);
+    if (rc == QSIM_HIP_OK) rc = rc2;
+  }
+  d->capturing = false;
+  if (rc != QSIM_HIP_OK) {
+    qsim_hip_graph_free(p);
+    return rc;
+  }
+  *plan = p;
+  return QSIM_HIP_OK;
 }
 
 }  // extern "C"

@@ -53,6 +53,25 @@ struct PassTimer {
 };
 PassTimer g_timer;
 
+// A recorded and captured sequence of passes (qsim_hip_graph_*).  While
+// recording, every matrix the passes stage gets a device buffer of its own
+// (the plan cache) instead of a slot of the shared ring; while capturing,
+// the same sequence takes those buffers back in order and stages nothing,
+// so the graph holds only kernel launches on device-resident data.
+struct GraphPlan {
+  std::vector<void*> bufs;
+  std::vector<size_t> sizes;
+  size_t next = 0;
+  hipStream_t stream = nullptr;
+  hipStream_t saved = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool mismatch = false;
+};
+enum PlanMode { kPlanOff, kPlanRecord, kPlanCapture };
+PlanMode g_plan_mode = kPlanOff;
+GraphPlan* g_plan = nullptr;
+
 }  // namespace
 
 void SetError(const char* fmt, ...) {
@@ -139,17 +158,39 @@ void* MatrixBuffer(size_t bytes) {
     return nullptr;
   }
   std::lock_guard<std::mutex> lock(g_mat_mu);
+  if (g_plan_mode == kPlanRecord) {
+    void* d = nullptr;
+    hipError_t e = hipMalloc(&d, bytes);
+    if (e != hipSuccess) {
+      HipFail(e, "hipMalloc(plan matrix)");
+      return nullptr;
+    }
+    g_plan->bufs.push_back(d);
+    g_plan->sizes.push_back(bytes);
+    return d;
+  }
+  if (g_plan_mode == kPlanCapture) {
+    GraphPlan& pl = *g_plan;
+    if (pl.next >= pl.bufs.size() || pl.sizes[pl.next] != bytes) {
+      pl.mismatch = true;
+      SetError("captured passes differ from the recorded ones");
+      return nullptr;
+    }
+    return pl.bufs[pl.next++];
+  }
   int slot = g_dev.mat_next;
   g_dev.mat_next = (slot + 1) % Device::kSlots;
   return g_dev.mat_dev + size_t(slot) * Device::kSlotBytes;
 }
 
 void PassTimerBegin() {
+  if (g_plan_mode == kPlanCapture) return;
   if (g_timer.on && g_timer.count < g_timer.capacity)
     (void)hipEventRecord(g_timer.ev[2 * g_timer.count], g_stream);
 }
 
 void PassTimerEnd() {
+  if (g_plan_mode == kPlanCapture) return;
   if (g_timer.on && g_timer.count < g_timer.capacity) {
     (void)hipEventRecord(g_timer.ev[2 * g_timer.count + 1], g_stream);
     ++g_timer.count;
@@ -158,6 +199,14 @@ void PassTimerEnd() {
 
 int StageMatrix(void* dev, const void* host, size_t bytes) {
   std::lock_guard<std::mutex> lock(g_mat_mu);
+  // Plan buffers are fresh: a blocking copy while recording (the host
+  // matrix may be a temporary), nothing while capturing (the recorded
+  // contents are the captured pass's).
+  if (g_plan_mode == kPlanRecord) {
+    QHIP_CHECK(hipMemcpy(dev, host, bytes, hipMemcpyHostToDevice));
+    return QSIM_HIP_OK;
+  }
+  if (g_plan_mode == kPlanCapture) return QSIM_HIP_OK;
   size_t slot = (static_cast<char*>(dev) - g_dev.mat_dev) / Device::kSlotBytes;
   char* h = g_dev.mat_host + slot * Device::kSlotBytes;
   // The pinned slot may still be the source of an earlier, unfinished copy.
@@ -706,6 +755,98 @@ int qsim_hip_event_elapsed_ms(int a, int b, float* ms) {
   }
   QHIP_CHECK(hipEventSynchronize(g_dev.ev[b]));
   QHIP_CHECK(hipEventElapsedTime(ms, g_dev.ev[a], g_dev.ev[b]));
+  return QSIM_HIP_OK;
+}
+
+int qsim_hip_graph_record_begin(void** plan) {
+  QHIP_TRY(EnsureInit());
+  if (plan == nullptr) {
+    SetError("null plan handle");
+    return QSIM_HIP_ERR_ARG;
+  }
+  if (g_plan_mode != kPlanOff) {
+    SetError("a pass sequence is already being recorded or captured");
+    return QSIM_HIP_ERR_ARG;
+  }
+  g_plan = new GraphPlan{};
+  g_plan_mode = kPlanRecord;
+  *plan = g_plan;
+  return QSIM_HIP_OK;
+}
+
+int qsim_hip_graph_record_end(void* plan) {
+  if (plan == nullptr || plan != g_plan || g_plan_mode != kPlanRecord) {
+    SetError("no recording in progress for this plan");
+    return QSIM_HIP_ERR_ARG;
+  }
+  g_plan_mode = kPlanOff;
+  g_plan = nullptr;
+  QHIP_CHECK(hipStreamSynchronize(g_stream));
+  return QSIM_HIP_OK;
+}
+
+int qsim_hip_graph_capture_begin(void* plan) {
+  auto* pl = static_cast<GraphPlan*>(plan);
+  if (pl == nullptr || g_plan_mode != kPlanOff || pl->exec != nullptr) {
+    SetError("plan is not ready for capture");
+    return QSIM_HIP_ERR_ARG;
+  }
+  QHIP_CHECK(hipStreamSynchronize(g_stream));
+  if (pl->stream == nullptr) QHIP_CHECK(hipStreamCreate(&pl->stream));
+  pl->next = 0;
+  pl->mismatch = false;
+  pl->saved = g_stream;
+  QHIP_CHECK(hipStreamBeginCapture(pl->stream, hipStreamCaptureModeRelaxed));
+  g_stream = pl->stream;
+  g_plan = pl;
+  g_plan_mode = kPlanCapture;
+  return QSIM_HIP_OK;
+}
+
+int qsim_hip_graph_capture_end(void* plan) {
+  auto* pl = static_cast<GraphPlan*>(plan);
+  if (pl == nullptr || pl != g_plan || g_plan_mode != kPlanCapture) {
+    SetError("no capture in progress for this plan");
+    return QSIM_HIP_ERR_ARG;
+  }
+  g_plan_mode = kPlanOff;
+  g_plan = nullptr;
+  g_stream = pl->saved;
+  hipGraph_t graph = nullptr;
+  QHIP_CHECK(hipStreamEndCapture(pl->stream, &graph));
+  pl->graph = graph;
+  if (pl->mismatch || pl->next != pl->bufs.size()) {
+    SetError("captured passes differ from the recorded ones");
+    return QSIM_HIP_ERR_ARG;
+  }
+  QHIP_CHECK(hipGraphInstantiate(&pl->exec, pl->graph, nullptr, nullptr, 0));
+  return QSIM_HIP_OK;
+}
+
+int qsim_hip_graph_launch(void* plan) {
+  auto* pl = static_cast<GraphPlan*>(plan);
+  if (pl == nullptr || pl->exec == nullptr) {
+    SetError("plan has no captured graph");
+    return QSIM_HIP_ERR_ARG;
+  }
+  QHIP_CHECK(hipGraphLaunch(pl->exec, g_stream));
+  return QSIM_HIP_OK;
+}
+
+int qsim_hip_graph_free(void* plan) {
+  auto* pl = static_cast<GraphPlan*>(plan);
+  if (pl == nullptr) return QSIM_HIP_OK;
+  if (pl == g_plan) {
+    SetError("plan is being recorded or captured");
+    return QSIM_HIP_ERR_ARG;
+  }
+  // Launches of the graph may still be in flight.
+  QHIP_CHECK(hipStreamSynchronize(g_stream));
+  if (pl->exec) QHIP_CHECK(hipGraphExecDestroy(pl->exec));
+  if (pl->graph) QHIP_CHECK(hipGraphDestroy(pl->graph));
+  if (pl->stream) QHIP_CHECK(hipStreamDestroy(pl->stream));
+  for (void* b : pl->bufs) QHIP_CHECK(hipFree(b));
+  delete pl;
   return QSIM_HIP_OK;
 }
 

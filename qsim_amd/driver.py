@@ -68,7 +68,9 @@ def driver_lib():
                 ("qsim_traj_state", i, [vp, u, vp]),
                 ("qsim_traj_free", None, [vp]),
                 ("qsim_driver_run_passes", i, [vp, vp, i, u, u, u]),
-                ("qsim_driver_run", i, [vp, vp, i, u])):
+                ("qsim_driver_run", i, [vp, vp, i, u]),
+                ("qsim_driver_capture", i, [vp, vp, i,
+                                            ctypes.POINTER(ctypes.c_void_p)])):
             f = getattr(d, name)
             f.restype = res
             f.argtypes = args
@@ -187,9 +189,40 @@ class Circuit:
         _check(driver_lib().qsim_driver_run_passes(
             self.h, state.ptr, state.prec, first, count, seed))
 
+    def capture(self, state: State) -> "Graph":
+        """Runs the circuit once on `state` and captures it as one HIP graph
+        with every matrix resident on the device (qsim_driver_capture)."""
+        p = ctypes.c_void_p()
+        _check(driver_lib().qsim_driver_capture(self.h, state.ptr, state.prec,
+                                                ctypes.byref(p)))
+        return Graph(p.value)
+
     def __del__(self):
         try:
             if self.h:
                 driver_lib().qsim_driver_free(self.h)
+        except Exception:
+            pass
+
+
+class Graph:
+    """A captured circuit (qsim_hip_graph_*): launch() applies the circuit
+    again to the state it was captured on, in one graph launch.  The graph
+    owns its matrices; it does not depend on the Circuit object."""
+
+    def __init__(self, handle: int):
+        self.h = handle
+
+    def launch(self):
+        _check(lib().qsim_hip_graph_launch(self.h))
+
+    def free(self):
+        if self.h:
+            _check(lib().qsim_hip_graph_free(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
         except Exception:
             pass
