@@ -6,7 +6,7 @@
 // bindings of pybind_interface/pybind_main.{h,cpp}.  This file supplies the
 // same Factory shape over this backend's SimulatorHIP / StateSpaceHIP and
 // compiles the same bindings (read from the qsim sources at build time), so
-// qsimcirq's `qsim_hip` import (`qsimcirq/__init__.py:34-39`) and every call
+// qsimcirq's `qsim_hip` import (`qsimcirq/__init__.py:34-42`) and every call
 // it makes (`qsim_simulate`, `qsim_simulate_fullstate`, samplers, expectation
 // values, qtrajectory and qsimh entry points) run here unchanged.
 
