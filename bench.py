@@ -384,6 +384,29 @@ def run_small_circuit(args):
             "graph_identical_to_eager": same}
 
 
+def stream_ceiling(n):
+    """GB/s of the backend's plainest in-place pass (ScaleKernel: each
+    complex64 amplitude read and written once, grid-stride) on an n-qubit
+    state: the practical ceiling a gate pass is held against, beside the
+    nominal 8 TB/s.  Best of 5 after a warm-up, from HIP events."""
+    import qsim_amd
+    ss = qsim_amd.StateSpace(np.float32)
+    st = ss.create(n)
+    ss.set_state_uniform(st)
+    ss.multiply(1.0, st)
+    best = float("inf")
+    for _ in range(5):
+        qsim_amd.Timer.record(0)
+        ss.multiply(1.0, st)
+        qsim_amd.Timer.record(1)
+        best = min(best, qsim_amd.Timer.elapsed_ms(0, 1))
+    st.free()
+    gbps = 16.0 * (1 << n) / (best * 1e-3) / 1e9
+    log(f"stream ceiling (ScaleKernel, {n}q): {best:.2f} ms = {gbps:.0f} GB/s")
+    return {"kernel": "ScaleKernel", "ms": best, "achieved": gbps,
+            "frac_of_peak": gbps / HBM_PEAK_GBPS}
+
+
 def trajectories_cpu(args, sizes):
     """qsim's QuantumTrajectorySimulator (oracle/_ref) on a sample of the
     batch: 4 trajectories per size, scaled to the batch's mix."""
@@ -499,6 +522,15 @@ def main():
         except Exception as e:  # reported beside the headline, never hidden
             log(f"trajectory batch failed: {e!r}")
             traj = {"error": repr(e)}
+
+    ceiling = None
+    try:
+        ceiling = stream_ceiling(args.qubits)
+        b["roofline"]["stream_ceiling"] = ceiling
+        b["roofline"]["frac_of_stream_ceiling"] = (
+            b["roofline"]["achieved"] / ceiling["achieved"])
+    except Exception as e:  # reported beside the headline, never hidden
+        log(f"stream ceiling failed: {e!r}")
 
     small = None
     if not args.no_small:

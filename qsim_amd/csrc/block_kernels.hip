@@ -29,6 +29,7 @@
 #include <algorithm>
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "internal.h"
@@ -212,8 +213,10 @@ struct TileIo {
     }
   }
 
-  // Issues the loads of the tile at `base` into v (this thread's elements).
-  template <bool FULL>
+  // Issues the loads of the tile at `base` into v (this thread's elements),
+  // those of the quarters of v whose bits are set in MASK (quarter c holds
+  // elements c PER/4 .. (c+1) PER/4 - 1).
+  template <bool FULL, unsigned MASK>
   __device__ __forceinline__ void Fetch(const float2* __restrict__ s,
                                         uint64_t base, const BlockGeom& g,
                                         float2 (&v)[PER]) const {
@@ -221,7 +224,9 @@ struct TileIo {
     const float2* p = s + base + tofs;
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
-      if (FULL || unsigned(i) < nper) v[i] = p[uint64_t(eofs[i]) << g.Lc];
+      if (((MASK >> (4 * i / PER)) & 1u) && (FULL || unsigned(i) < nper)) {
+        v[i] = p[uint64_t(eofs[i]) << g.Lc];
+      }
     }
   }
 
@@ -266,9 +271,13 @@ struct TileIo {
 // every tile, so the ring wraps from the last op to the first.  One wave
 // per SIMD per workgroup leaves each wave 256 registers: enough to keep two
 // batches of an op in flight and the next tile's amplitudes (64 registers)
-// in flight from HBM during the last op and the store.  Vector memory
-// counters complete in order, so that prefetch is issued after the last
-// op's record fetch and never delays a record.
+// in flight from HBM while this tile's ops run.  Vector memory counters
+// complete in order, so the prefetch goes out in quarters, each right after
+// an op's record fetch: the record an op waits for is never behind a
+// quarter issued during that op, and each quarter has at least one whole op
+// to arrive (quarters 0..2 with ops 0..2, the rest with the last op).  The
+// op sequence is straight-line per program length so that the compiler's
+// wait counts stay exact.
 // FULL: the tile holds 2^13 amplitudes, every thread's 32 elements.
 template <bool FULL>
 __global__ void __launch_bounds__(kThreads)
@@ -323,8 +332,8 @@ GateBlockKernel(float2* __restrict__ s, BlockGeom g, BlockProgram p,
   float2 buf[PER];
   uint64_t tile = blockIdx.x;
   if (tile < g.ntiles) {
-    io.template Fetch<FULL>(s, InsertZeros(tile << g.Lc, g.outer, g.nouter), g,
-                            buf);
+    io.template Fetch<FULL, 15u>(
+        s, InsertZeros(tile << g.Lc, g.outer, g.nouter), g, buf);
     io.template Put<FULL>(g, buf, lds);
   }
   __syncthreads();
@@ -336,9 +345,38 @@ GateBlockKernel(float2* __restrict__ s, BlockGeom g, BlockProgram p,
     // in-bounds read that is never used), so the prefetch is unconditional.
     const uint64_t next = tile + gridDim.x < g.ntiles ? tile + gridDim.x : tile;
     const uint64_t nbase = InsertZeros(next << g.Lc, g.outer, g.nouter);
-    for (unsigned op = 0; op + 1 < p.nops; ++op) run_op(op, op + 1, [] {});
-    run_op(p.nops - 1, 0,
-           [&] { io.template Fetch<FULL>(s, nbase, g, buf); });
+    auto fetch = [&](auto mask) {
+      return [&, mask] {
+        io.template Fetch<FULL, decltype(mask)::value>(s, nbase, g, buf);
+      };
+    };
+    using Q0 = std::integral_constant<unsigned, 1u>;
+    using Q1 = std::integral_constant<unsigned, 2u>;
+    using Q2 = std::integral_constant<unsigned, 4u>;
+    // One op, and partial tiles (fewer elements per thread, guarded):
+    // everything during the last op.
+    switch (FULL ? p.nops : 1u) {
+      case 1:
+        for (unsigned op = 0; op + 1 < p.nops; ++op) run_op(op, op + 1, [] {});
+        run_op(p.nops - 1, 0, fetch(std::integral_constant<unsigned, 15u>{}));
+        break;
+      case 2:
+        run_op(0, 1, fetch(Q0{}));
+        run_op(1, 0, fetch(std::integral_constant<unsigned, 14u>{}));
+        break;
+      case 3:
+        run_op(0, 1, fetch(Q0{}));
+        run_op(1, 2, fetch(Q1{}));
+        run_op(2, 0, fetch(std::integral_constant<unsigned, 12u>{}));
+        break;
+      default:
+        run_op(0, 1, fetch(Q0{}));
+        run_op(1, 2, fetch(Q1{}));
+        run_op(2, 3, fetch(Q2{}));
+        for (unsigned op = 3; op + 1 < p.nops; ++op) run_op(op, op + 1, [] {});
+        run_op(p.nops - 1, 0, fetch(std::integral_constant<unsigned, 8u>{}));
+        break;
+    }
     io.template Store<16, FULL>(s, base, g, lds);
     __syncthreads();
     io.template Put<FULL>(g, buf, lds);
