@@ -13,6 +13,8 @@ apply loop over the gfx950 kernels.  The paths run on the same circuit:
   blocked-13         cache-blocked passes over 13 tile qubits (5 lowest + 8
                      chosen), gates fused to <= 4 qubits in FP64 inside each
                      pass, split-BF16 MFMA on the LDS-resident tile
+  blocked-14         the same over 14 tile qubits (5 lowest + 9 chosen; one
+                     128 KiB tile per CU, fewer passes)
   learned            the partition the learned planner picks for this
                      circuit with the committed cost model
                      (qsim_amd/data/cost_model.json), among qsim's fuser at
@@ -62,6 +64,7 @@ PATHS = {
     "fp32-mfma": ("mqubit", 4, False, 0),
     "bf16-k6": ("mqubit", 6, True, 2),
     "blocked-13": ("blocked", 13, 5, 0),
+    "blocked-14": ("blocked", 14, 5, 0),
     # the learned planner's pick (qsim_amd/data/cost_model.json)
     "learned": ("learned", 0, 0, 0),
 }
@@ -69,7 +72,7 @@ PATHS = {
 # HBM bytes per amplitude per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE
 # runs, scaled by what the counters read for a copy of known size (reads x 2,
 # writes x 1); profiles/gate_passes_32q_blocked_vs_fused.md.
-MEASURED_HBM_BYTES_PER_AMP = {"GateBlockKernel": 16.06}
+MEASURED_HBM_BYTES_PER_AMP = {"GateBlockKernel": 16.04}  # 14-qubit tiles
 
 # One 32-group batch of a 4-qubit op in a blocked pass: 8 MFMAs of
 # 32 x 32 x 16 (2 K blocks x 4 split products).
@@ -227,9 +230,11 @@ def run_path(args, text, name):
     }
 
 
-def fidelities(args, names):
+def fidelities(args, names, picks=None):
     """1 - F (normalised states) and the norm drift of each path at
-    --fidelity-qubits qubits."""
+    --fidelity-qubits qubits.  A path with a planner pick in `picks` (the
+    learned planner's choice at the benchmark size) is measured with that
+    same partition kind, not re-planned at the smaller size."""
     import qsim_amd
     from qsim_amd import circuits, driver
 
@@ -246,7 +251,12 @@ def fidelities(args, names):
     tmp = ss64.create(nq)
     for name in names:
         c = driver.Circuit(text)
-        _, method, _ = fuse(c, name)
+        pick = (picks or {}).get(name)
+        if pick is not None:
+            c.apply_candidate(pick)
+            method = pick["method"]
+        else:
+            _, method, _ = fuse(c, name)
         qsim_amd.set_gate_method(method)
         st = ss32.create(nq)
         try:
@@ -487,7 +497,9 @@ def main():
 
     fid = {}
     if not args.no_fidelity:
-        fid = fidelities(args, names)
+        fid = fidelities(args, names,
+                         {p: results[p]["planner_pick"] for p in names
+                          if results[p].get("planner_pick")})
         for p in names:
             results[p]["fidelity"] = fid[p]
 

@@ -52,7 +52,7 @@ struct BlockPlan {
 };
 
 struct BlockPlannerParameter {
-  unsigned tile_bits = 13;  // tile qubits per pass (9..13)
+  unsigned tile_bits = 13;  // tile qubits per pass (9..14)
   unsigned low_bits = 5;    // lowest qubits always in the tile (>= 3)
   unsigned max_ops = 32;    // fused gates per pass
 };
@@ -265,7 +265,7 @@ bool PlanBlocks(const BlockPlannerParameter& param, unsigned num_qubits,
                 const std::vector<Operation<FP>>& ops, BlockPlan& plan) {
   plan = BlockPlan{};
   const unsigned tb = std::min(param.tile_bits, num_qubits);
-  if (tb < 9 || tb > 13) return false;
+  if (tb < 9 || tb > 14) return false;
   const unsigned lc = std::min(std::max(param.low_bits, 3u), tb);
 
   size_t i = 0;

@@ -209,7 +209,7 @@ int qsim_hip_apply_controlled_gate(int prec, void* state, unsigned num_qubits,
 
 /* A cache-blocked pass: applies num_gates (<= 32) gates in order with one
  * stream of the state through HBM.  The state is cut into tiles over the
- * tile qubits (ascending, 9..13 of them, starting with the run 0, 1, 2, ...
+ * tile qubits (ascending, 9..14 of them, starting with the run 0, 1, 2, ...
  * of at least 3 qubits, at most 10 beyond it); gate i acts on
  * gate_num_qubits[i] (1..4) ascending qubits gate_qubits[4 i ...], all tile
  * qubits, with the next 2 * 4^k matrix elements of `matrices` (row-major

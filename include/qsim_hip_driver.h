@@ -35,7 +35,7 @@ unsigned qsim_driver_num_gates(void* circuit);
 int qsim_driver_fuse(void* circuit, unsigned max_fused_size, int fuser,
                      int matrix_prec);
 
-/* Partitions the circuit into cache-blocked passes of `tile_bits` (9..13)
+/* Partitions the circuit into cache-blocked passes of `tile_bits` (9..14)
  * tile qubits that always include the `low_bits` (3..tile_bits) lowest
  * qubits (qsim_amd/block_planner.h); the gates of a pass are fused to
  * <= 4 qubits by MultiQubitGateFuser with FP64 matrices.  Operations that

@@ -363,7 +363,7 @@ int qsim_driver_fuse(void* c, unsigned max_fused_size, int fuser,
 
 int qsim_driver_fuse_blocked(void* c, unsigned tile_bits, unsigned low_bits) {
   auto* d = static_cast<DriverCircuit*>(c);
-  if (tile_bits < 9 || tile_bits > 13 || low_bits < 3 ||
+  if (tile_bits < 9 || tile_bits > 14 || low_bits < 3 ||
       low_bits > tile_bits) {
     return QSIM_HIP_ERR_ARG;
   }

@@ -2,7 +2,7 @@
 //
 // A plain gate pass streams the whole state through HBM once per fused gate
 // (16 B per amplitude).  A blocked pass streams it once for a whole run of
-// gates: the state is cut into tiles of 2^tb amplitudes (tb <= 13, 64 KiB of
+// gates: the state is cut into tiles of 2^tb amplitudes (tb <= 14, 128 KiB of
 // complex64) spanned by the Lc lowest qubits (contiguous runs in memory) and
 // h = tb - Lc chosen higher qubits; each workgroup stages a tile in LDS,
 // applies every gate of the program to it there, and writes it back.  Any
@@ -19,10 +19,12 @@
 // gfx950): a[e] = A[l%32][8(l/32) + e], b[e] = B[8(l/32) + e][l%32],
 // C register r of lane l = C[8(r/4) + 4(l/32) + r%4][l%32].
 //
-// The grid is a tile-stride loop; 512 threads (8 waves) share a tile, and
-// the 64 KiB tile (plus a 16 KiB ring of op records) lets two workgroups share a
-// CU's 160 KiB of LDS, so one workgroup's HBM phase overlaps the other's
-// MFMA phase.
+// The grid is a tile-stride loop.  Tiles of up to 13 qubits: 256 threads
+// share a tile, and the 64 KiB tile (plus a 16 KiB ring of op records) lets
+// two workgroups share a CU's 160 KiB of LDS, so one workgroup's HBM phase
+// overlaps the other's MFMA phase.  14-qubit tiles: 512 threads, one
+// workgroup per CU (128 + 16 KiB); fewer passes for a circuit, with the
+// overlap coming from the next tile's prefetch alone.
 
 #include <hip/hip_runtime.h>
 
@@ -44,9 +46,16 @@ typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
 typedef float v2f __attribute__((ext_vector_type(2)));
 
-constexpr int kThreads = 256;
-constexpr int kWaves = kThreads / 64;
-constexpr size_t kTileBytes = (size_t{1} << kBlockMaxBits) * sizeof(float2);
+// Kernel shapes by tile qubits TB: up to 13 (64 KiB tiles), two 256-thread
+// workgroups per CU; 14 (128 KiB), one 512-thread workgroup per CU.  Both
+// give every thread 32 tile elements and every wave 256 registers.
+template <int TB>
+struct BlockShape {
+  static_assert(TB == 13 || TB == 14, "tile shape");
+  static constexpr int NT = TB == 13 ? 256 : 512;
+  static constexpr int kWaves = NT / 64;
+  static constexpr size_t kTileBytes = (size_t{1} << TB) * sizeof(float2);
+};
 // BF16 fragment elements of one 4-qubit op: 2 K blocks x (hi, mid, lo) x 512.
 constexpr size_t kOpFrag = 2 * 3 * 512;
 // One op record in 32-bit words: the fragments (1536 words); the member
@@ -59,9 +68,10 @@ constexpr int kTableWords = 32;
 constexpr int kLaneWords = 32;
 constexpr int kOpWords = 2048;
 static_assert(kFragWords + kTableWords + kLaneWords <= kOpWords, "op record");
-constexpr int kRingQuads = kOpWords / 4 / kThreads;  // uint4 per thread per op
-static_assert(kOpWords % (4 * kThreads) == 0, "fragment ring split");
-constexpr size_t kBlockLds = kTileBytes + 2 * kOpWords * sizeof(uint32_t);
+template <int TB>
+constexpr size_t BlockLds() {
+  return BlockShape<TB>::kTileBytes + 2 * kOpWords * sizeof(uint32_t);
+}
 
 // The tile at LDS byte offset `b`: the kernel has no static LDS, so its
 // dynamic block starts at offset 0 (checked on the host before launch), and
@@ -83,11 +93,11 @@ __device__ __forceinline__ LdsV2f* LdsAt(unsigned b) {
 // and member offsets occupy disjoint bits, so a member's slot is the XOR of
 // the three parts' swizzled slots: the batch part is scalar work and an
 // address is one XOR (the tile sits at LDS offset 0).
-__device__ __forceinline__ unsigned BatchSlot(unsigned bt, unsigned b0,
-                                              unsigned b1, unsigned b2,
-                                              unsigned b3) {
-  const unsigned pb = ((bt & 1u) << b0) | (((bt >> 1) & 1u) << b1) |
-                      (((bt >> 2) & 1u) << b2) | (((bt >> 3) & 1u) << b3);
+__device__ __forceinline__ unsigned BatchSlot(unsigned bt,
+                                              const unsigned (&b)[5]) {
+  unsigned pb = 0;
+#pragma unroll
+  for (int i = 0; i < 5; ++i) pb |= ((bt >> i) & 1u) << b[i];
   return TileSwz(pb) * unsigned(sizeof(float2));
 }
 
@@ -140,13 +150,13 @@ __device__ __forceinline__ void ApplyTileOp(const uint8_t* bp,
                                             const uint32_t (&wr)[8],
                                             unsigned nbatch, unsigned wave,
                                             unsigned nwaves) {
-  const unsigned b0 = bp[0], b1 = bp[1], b2 = bp[2], b3 = bp[3];
+  const unsigned b[5] = {bp[0], bp[1], bp[2], bp[3], bp[4]};
   // Two batches at a time (their groups are disjoint): batch 1's reads and
   // split overlap batch 0's MFMA chain.
   unsigned bt = wave;
   for (; bt + nwaves < nbatch; bt += 2 * nwaves) {
-    const unsigned sp0 = BatchSlot(bt, b0, b1, b2, b3);
-    const unsigned sp1 = BatchSlot(bt + nwaves, b0, b1, b2, b3);
+    const unsigned sp0 = BatchSlot(bt, b);
+    const unsigned sp1 = BatchSlot(bt + nwaves, b);
     v2f v0[8], v1[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) v0[q] = *LdsAt(sp0 ^ rd[q]);
@@ -161,7 +171,7 @@ __device__ __forceinline__ void ApplyTileOp(const uint8_t* bp,
     WriteBatch(sp1, wr, c1);
   }
   if (bt < nbatch) {
-    const unsigned sp0 = BatchSlot(bt, b0, b1, b2, b3);
+    const unsigned sp0 = BatchSlot(bt, b);
     v2f v0[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) v0[q] = *LdsAt(sp0 ^ rd[q]);
@@ -263,13 +273,13 @@ struct TileIo {
   }
 };
 
-// Two 256-thread workgroups per CU, each with a 64 KiB tile and a two-slot
-// ring of op records in LDS: while op i runs from slot i % 2, each thread
-// holds its 32 bytes of op i + 1's record in flight from global memory
-// (L2) and writes them to the other slot before the op's barrier, so the
-// record latency hides behind the op's MFMA work.  The program repeats for
-// every tile, so the ring wraps from the last op to the first.  One wave
-// per SIMD per workgroup leaves each wave 256 registers: enough to keep two
+// Each workgroup has its tile and a two-slot ring of op records in LDS:
+// while op i runs from slot i % 2, each thread holds its share (32 or 16
+// bytes) of op i + 1's record in flight from global memory (L2) and writes
+// it to the other slot before the op's barrier, so the record latency hides
+// behind the op's MFMA work.  The program repeats for every tile, so the
+// ring wraps from the last op to the first.  Both shapes (BlockShape) run
+// two waves per SIMD, so each wave has 256 registers: enough to keep two
 // batches of an op in flight and the next tile's amplitudes (64 registers)
 // in flight from HBM while this tile's ops run.  Vector memory counters
 // complete in order, so the prefetch goes out in quarters, each right after
@@ -278,36 +288,41 @@ struct TileIo {
 // to arrive (quarters 0..2 with ops 0..2, the rest with the last op).  The
 // op sequence is straight-line per program length so that the compiler's
 // wait counts stay exact.
-// FULL: the tile holds 2^13 amplitudes, every thread's 32 elements.
-template <bool FULL>
-__global__ void __launch_bounds__(kThreads)
+// TB: the kernel's tile shape; FULL: the tile holds 2^TB amplitudes, every
+// thread's 32 elements (else fewer, 2^tb with tb < TB).
+template <int TB, bool FULL>
+__global__ void __launch_bounds__(BlockShape<TB>::NT)
 __attribute__((amdgpu_waves_per_eu(2)))
 GateBlockKernel(float2* __restrict__ s, BlockGeom g, BlockProgram p,
                 const uint16_t* __restrict__ frag) {
+  constexpr int NT = BlockShape<TB>::NT;
+  constexpr int kRingQuads = kOpWords / 4 / NT;  // uint4 per thread per op
+  static_assert(kOpWords % (4 * NT) == 0, "record split");
   extern __shared__ char lds_raw[];
   float2* lds = reinterpret_cast<float2*>(lds_raw);
-  uint32_t* ring = reinterpret_cast<uint32_t*>(lds_raw + kTileBytes);
+  uint32_t* ring =
+      reinterpret_cast<uint32_t*>(lds_raw + BlockShape<TB>::kTileBytes);
   const int l = threadIdx.x & 63;
   const unsigned wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const unsigned nbatch = (1u << g.tb) >> 9;  // 32 groups of 16 per batch
-  constexpr int PER = (1 << kBlockMaxBits) / kThreads;
+  constexpr int PER = (1 << TB) / NT;
+  static_assert(PER == 32, "prefetch quarters");
   const uint4* gfrag = reinterpret_cast<const uint4*>(frag);
   uint4* ring4 = reinterpret_cast<uint4*>(ring);
   // Op 0's record into slot 0.
 #pragma unroll
   for (int w = 0; w < kRingQuads; ++w) {
-    ring4[threadIdx.x + kThreads * w] = gfrag[threadIdx.x + kThreads * w];
+    ring4[threadIdx.x + NT * w] = gfrag[threadIdx.x + NT * w];
   }
-  TileIo<PER, kThreads> io;
+  TileIo<PER, NT> io;
   io.Init(g);
   unsigned k = 0;  // ops run so far: the current op's slot is k % 2
   // Op `op` on the tile in LDS; `extra` runs once op nxt's record loads are
   // issued.
   auto run_op = [&](unsigned op, unsigned nxt, auto&& extra) {
       const uint4* gn = gfrag + size_t(nxt) * (kOpWords / 4);
-      static_assert(kRingQuads == 2, "record split");
       const uint4 pre0 = gn[threadIdx.x];
-      const uint4 pre1 = gn[threadIdx.x + kThreads];
+      const uint4 pre1 = kRingQuads > 1 ? gn[threadIdx.x + NT] : uint4{};
       extra();
       const uint32_t* cur = ring + (k & 1u) * kOpWords;
       const v8bf* fp = reinterpret_cast<const v8bf*>(cur);
@@ -322,10 +337,10 @@ GateBlockKernel(float2* __restrict__ s, BlockGeom g, BlockProgram p,
         rd[q] = tab[q] ^ sj;
         wr[q] = tab[8 + q] ^ sj;
       }
-      ApplyTileOp(p.bpos[op], f, rd, wr, nbatch, wave, kWaves);
+      ApplyTileOp(p.bpos[op], f, rd, wr, nbatch, wave, BlockShape<TB>::kWaves);
       uint4* slot = ring4 + ((k + 1) & 1u) * (kOpWords / 4);
       slot[threadIdx.x] = pre0;
-      slot[threadIdx.x + kThreads] = pre1;
+      if (kRingQuads > 1) slot[threadIdx.x + NT] = pre1;
       __syncthreads();
       ++k;
   };
@@ -442,7 +457,7 @@ void ChooseLanePositions(const unsigned* pad, unsigned tb, unsigned* lane,
   for (unsigned q = 0; q < np; ++q) {
     if (std::find(lane, lane + 5, nonk[q]) == lane + 5) batch[nb++] = nonk[q];
   }
-  for (; nb < 4; ++nb) batch[nb] = 0;
+  for (; nb < kBlockMaxBits - 9; ++nb) batch[nb] = 0;
 }
 
 // U (2^k x 2^k over positions `pos`, ascending) embedded into the 16 x 16
@@ -509,12 +524,14 @@ int PrepareGateBlock(unsigned n, const unsigned* tile, unsigned nt,
              "and add at most %u higher ones", 2u, kBlockMaxHigh);
     return QSIM_HIP_ERR_ARG;
   }
-  // The kernel's staging keeps the contiguous run within its 8 thread bits;
-  // further low qubits are handled as tile qubits above the run (the same
-  // tile-local positions and addresses).
-  if (g.Lc > 8) {
-    g.Lc = 8;
-    g.h = nt - 8;
+  // The kernel's staging keeps the contiguous run within its thread bits
+  // (8 for the 256-thread shape, 9 for the 512-thread one); further low
+  // qubits are handled as tile qubits above the run (the same tile-local
+  // positions and addresses).
+  const unsigned max_lc = nt == 14 ? 9 : 8;
+  if (g.Lc > max_lc) {
+    g.Lc = max_lc;
+    g.h = nt - max_lc;
   }
   g.nouter = g.h;
   for (unsigned b = 0; b < g.h; ++b) {
@@ -588,13 +605,13 @@ int PrepareGateBlock(unsigned n, const unsigned* tile, unsigned nt,
       }
     }
     // Group table: lane j's group has j's bits at the lane positions.
-    unsigned lane[5], batch[4];
+    unsigned lane[5], batch[kBlockMaxBits - 9];
     ChooseLanePositions(pad, nt, lane, batch);
     for (unsigned j = 0; j < 32; ++j) {
       tab[kTableWords + j] =
           uint32_t(sizeof(float2)) * HostTileSwz(Deposit(j, lane, 5));
     }
-    for (int b = 0; b < 4; ++b) p.bpos[i][b] = uint8_t(batch[b]);
+    for (unsigned b = 0; b < kBlockMaxBits - 9; ++b) p.bpos[i][b] = uint8_t(batch[b]);
     std::memcpy(rec + 2 * size_t(kFragWords), tab, sizeof(tab));
   }
 
@@ -610,15 +627,20 @@ int RunGateBlock(float2* s, const BlockGeom& g, const BlockProgram& p,
   static hipError_t attr = hipSuccess;
   static size_t static_lds = 0;
   std::call_once(once, [] {
-    const void* fns[2] = {reinterpret_cast<const void*>(&GateBlockKernel<true>),
-                          reinterpret_cast<const void*>(&GateBlockKernel<false>)};
-    for (const void* fn : fns) {
+    const struct {
+      const void* fn;
+      size_t lds;
+    } fns[3] = {
+        {reinterpret_cast<const void*>(&GateBlockKernel<13, true>), BlockLds<13>()},
+        {reinterpret_cast<const void*>(&GateBlockKernel<13, false>), BlockLds<13>()},
+        {reinterpret_cast<const void*>(&GateBlockKernel<14, true>), BlockLds<14>()}};
+    for (const auto& f : fns) {
       hipFuncAttributes fa{};
-      if (attr == hipSuccess) attr = hipFuncGetAttributes(&fa, fn);
+      if (attr == hipSuccess) attr = hipFuncGetAttributes(&fa, f.fn);
       static_lds = std::max(static_lds, fa.sharedSizeBytes);
       if (attr == hipSuccess) {
-        attr = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   int(kBlockLds));
+        attr = hipFuncSetAttribute(
+            f.fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(f.lds));
       }
     }
   });
@@ -627,16 +649,26 @@ int RunGateBlock(float2* s, const BlockGeom& g, const BlockProgram& p,
     SetError("blocked-pass kernel has %zu bytes of static LDS", static_lds);
     return QSIM_HIP_ERR_RUNTIME;
   }
-  // Two workgroups per CU fit in LDS (2 x 80 KiB); a few tiles each.
-  const uint64_t blocks =
-      std::min<uint64_t>(g.ntiles, uint64_t(NumCUs()) * 2 * 8);
   PassTimerBegin();
-  if (g.tb == kBlockMaxBits) {
-    hipLaunchKernelGGL(GateBlockKernel<true>, dim3(unsigned(blocks)),
-                       dim3(kThreads), kBlockLds, Stream(), s, g, p, dfrag);
+  if (g.tb == 14) {
+    // One workgroup per CU fits in LDS (144 KiB); a long tile loop each.
+    const uint64_t blocks = std::min<uint64_t>(g.ntiles, uint64_t(NumCUs()) * 4);
+    hipLaunchKernelGGL((GateBlockKernel<14, true>), dim3(unsigned(blocks)),
+                       dim3(BlockShape<14>::NT), BlockLds<14>(), Stream(), s,
+                       g, p, dfrag);
   } else {
-    hipLaunchKernelGGL(GateBlockKernel<false>, dim3(unsigned(blocks)),
-                       dim3(kThreads), kBlockLds, Stream(), s, g, p, dfrag);
+    // Two workgroups per CU fit in LDS (2 x 80 KiB); a few tiles each.
+    const uint64_t blocks =
+        std::min<uint64_t>(g.ntiles, uint64_t(NumCUs()) * 2 * 8);
+    if (g.tb == 13) {
+      hipLaunchKernelGGL((GateBlockKernel<13, true>), dim3(unsigned(blocks)),
+                         dim3(BlockShape<13>::NT), BlockLds<13>(), Stream(), s,
+                         g, p, dfrag);
+    } else {
+      hipLaunchKernelGGL((GateBlockKernel<13, false>), dim3(unsigned(blocks)),
+                         dim3(BlockShape<13>::NT), BlockLds<13>(), Stream(), s,
+                         g, p, dfrag);
+    }
   }
   PassTimerEnd();
   QHIP_LAUNCHED("GateBlockKernel");

@@ -109,10 +109,10 @@ struct TileGeom {
 // of up to kBlockMaxOps 4-qubit gates on tile-local positions runs on it
 // before it is written back.  Tile index -> base = (tile << Lc) with zero
 // bits inserted at the high qubits.
-constexpr unsigned kBlockMaxHigh = 10;
+constexpr unsigned kBlockMaxHigh = 11;
 constexpr unsigned kBlockMaxOps = 32;
 constexpr unsigned kBlockMinBits = 9;   // one 32-group batch of 16 members
-constexpr unsigned kBlockMaxBits = 13;  // 64 KiB of complex64 per tile
+constexpr unsigned kBlockMaxBits = 14;  // 128 KiB of complex64 per tile
 
 struct BlockGeom {
   uint64_t ntiles;
@@ -129,7 +129,7 @@ struct BlockProgram {
   uint8_t tpos[kBlockMaxOps][4];  // tile-local targets of each op, ascending
   // Tile-local positions that carry the batch index of each op's groups
   // (tb - 9 of them; unused entries are 0 and only ever see zero bits).
-  uint8_t bpos[kBlockMaxOps][4];
+  uint8_t bpos[kBlockMaxOps][kBlockMaxBits - 9];
 };
 
 __host__ __device__ inline uint64_t InsertZeros(uint64_t i, const unsigned* pos,

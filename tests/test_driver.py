@@ -58,7 +58,7 @@ def _run_blocked_fp64(c, n):
 
 
 @pytest.mark.parametrize("tile_bits,low_bits", [(13, 5), (11, 3), (9, 4),
-                                                (12, 7)])
+                                                (12, 7), (14, 5)])
 def test_blocked_plan_structure(tile_bits, low_bits):
     text = circuits.grid_circuit(20, 14, seed=0)
     c = driver.Circuit(text)

@@ -59,7 +59,9 @@ def run_case(sim, n, tile, gates, rng):
 
 @pytest.mark.parametrize("n,lc,tb", [(9, 9, 9), (12, 5, 9), (13, 13, 13),
                                      (14, 3, 13), (16, 5, 13), (16, 7, 12),
-                                     (18, 6, 11), (20, 5, 13), (22, 8, 13)])
+                                     (18, 6, 11), (20, 5, 13), (22, 8, 13),
+                                     (14, 14, 14), (17, 5, 14), (20, 9, 14),
+                                     (22, 3, 14)])
 def test_block_random_programs(sim, n, lc, tb):
     rng = np.random.default_rng(1000 * n + 10 * lc + tb)
     for trial in range(3):
@@ -106,7 +108,7 @@ def test_block_matches_gate_passes_on_grid_layers(sim):
 
 
 @pytest.mark.parametrize("bad", ["outside", "unsorted", "k5", "too_many",
-                                 "short_run", "small_tile"])
+                                 "short_run", "small_tile", "big_tile"])
 def test_block_rejects_bad_programs(sim, bad):
     rng = np.random.default_rng(3)
     n = 16
@@ -126,6 +128,9 @@ def test_block_rejects_bad_programs(sim, bad):
         gates = [([3], to_interleaved(random_matrix(1, rng), np.float64))]
     elif bad == "small_tile":
         tile = list(range(8))
+        gates = [([3], to_interleaved(random_matrix(1, rng), np.float64))]
+    elif bad == "big_tile":
+        tile = list(range(15))
         gates = [([3], to_interleaved(random_matrix(1, rng), np.float64))]
     st = upload(random_state(n, rng), n)
     before = st.to_numpy()
