@@ -319,8 +319,9 @@ def run_trajectories(args):
     b = trajectories.TrajectoryBatch(sizes, 12, p_depol=1e-3, gamma_damp=5e-3)
     circuit_of = [i % 3 for i in range(ntraj)]
     reps = list(range(ntraj))
-    plan_s = b.plan(circuit_of, reps, 4)
-    threads, kmax, tune = b.autotune(sample=96, log=log)
+    b.plan(circuit_of, reps, 4)
+    threads, plan, tune = b.autotune(log=log)
+    plan_s = b.plan(circuit_of, reps, plan)  # host planning of the pick
     b.run(threads, use_graph=True)
     times = []
     for _ in range(3):
@@ -329,16 +330,17 @@ def run_trajectories(args):
         times.append(time.time() - t0)
     best = min(times)
     zz_mean = float(np.mean([np.mean(b.zz(t)) for t in range(ntraj)]))
+    kernel = "tiled" if threads == trajectories.TILED else f"{threads} threads"
     log(f"trajectories: {ntraj} in {best * 1e3:.1f} ms "
-        f"({ntraj / best:.0f}/s), threads {threads}, k {kmax}, host plan "
-        f"{plan_s:.2f}s")
+        f"({ntraj / best:.0f}/s), {kernel}, "
+        f"{trajectories.describe_plan(plan)}, host plan {plan_s:.2f}s")
     out = {"workload": "grid circuit 18/20/22 qubits, 12 cycles, "
                        "depolarise 1e-3 + amplitude damping 5e-3",
            "trajectories": ntraj, "batch_ms": best * 1e3,
            "trajectories_per_s": ntraj / best,
            "trajectories_per_s_with_host_planning": ntraj / (best + plan_s),
-           "threads": threads,
-           "max_fused_size": kmax, "host_plan_s": plan_s,
+           "kernel": kernel, "plan": trajectories.describe_plan(plan),
+           "host_plan_s": plan_s,
            "autotune": tune, "mean_zz": zz_mean}
     b.free()
     if not args.no_cpu:

@@ -307,7 +307,21 @@ int qsim_hip_pass_timer_stop(float* ms, unsigned max_count, unsigned* count);
  * a channel's Kraus operator is chosen from the draw against probability
  * bounds, and against the norms of the updated state when the bounds do not
  * decide (lib/qtrajectory.h:296-356). */
-enum { QSIM_HIP_TRAJ_GATE = 0, QSIM_HIP_TRAJ_DAMP_DECIDE = 1 };
+enum {
+  QSIM_HIP_TRAJ_GATE = 0,
+  QSIM_HIP_TRAJ_DAMP_DECIDE = 1,
+  /* The header of a tile pass over the k gate operations that follow it
+   * (1 <= k <= 32): q[0] | q[1] << 32 is the mask of its tile qubits (9..14
+   * of them, starting with qubits 0, 1, 2), and every gate lies inside the
+   * tile.  The tiled kernel applies the k gates with one stream of the
+   * state (tiles staged in LDS, split-BF16 MFMA, as
+   * qsim_hip_apply_gate_block does); the per-gate kernels ignore the header
+   * and apply the gates one by one.  moff is not read. */
+  QSIM_HIP_TRAJ_TILE_PASS = 2
+};
+/* qsim_hip_traj_run's `threads` value for the tiled kernel: 256 threads
+ * per trajectory, or 512 when a tile pass has 14 qubits. */
+enum { QSIM_HIP_TRAJ_TILED = 1 };
 
 typedef struct {
   uint32_t kind;   /* QSIM_HIP_TRAJ_* */
@@ -341,14 +355,17 @@ int qsim_hip_traj_prepare(const qsim_hip_traj_desc* trajs, unsigned num_traj,
                           const uint32_t* pairs, uint32_t num_pairs,
                           uint32_t num_decisions, void** handle);
 /* One launch of the whole batch with `threads` (256, 512 or 1024) per
- * trajectory; with use_graph the launch is captured once in a hipGraph and
- * replayed after.  Waits for completion. */
+ * trajectory, or with the tiled kernel (QSIM_HIP_TRAJ_TILED); with
+ * use_graph the launch is captured once in a hipGraph and replayed after.
+ * Waits for completion. */
 int qsim_hip_traj_run(void* handle, unsigned threads, int use_graph);
 /* <Z_a Z_b> of every pair (FP64, pair order of the batch) and the decision
  * record (0: K0, 1: K1, -1: not reached). */
 int qsim_hip_traj_results(void* handle, double* zz, uint32_t num_pairs,
                           int32_t* decisions, uint32_t num_decisions);
-/* Final state of trajectory t (complex64, 2^n pairs) to host memory. */
+/* Final state of trajectory t (complex64, 2^n pairs) to host memory,
+ * normalised when the descriptor asks for it (the scaling by the norm of
+ * the last run is applied to the copy). */
 int qsim_hip_traj_state(void* handle, unsigned t, float* out);
 int qsim_hip_traj_free(void* handle);
 

@@ -121,6 +121,15 @@ unsigned qsim_traj_num_channels(void* batch, unsigned circuit);
 int qsim_traj_plan(void* batch, const unsigned* circuit_of,
                    const uint64_t* reps, unsigned num_traj,
                    unsigned max_fused_size);
+/* The same sampling with every run of deferred gates partitioned into tile
+ * passes (QSIM_HIP_TRAJ_TILE_PASS) of tile_bits (9..14) qubits, low_bits
+ * (3..9) of them the lowest, by the blocked-pass planner; states under 9
+ * qubits keep fused gates of at most 4 qubits.  Run such a batch with
+ * threads = QSIM_HIP_TRAJ_TILED (the per-gate kernels also accept it). */
+int qsim_traj_plan_tiled(void* batch, const unsigned* circuit_of,
+                         const uint64_t* reps, unsigned num_traj,
+                         unsigned tile_bits, unsigned low_bits);
+/* threads: 256, 512, 1024 (per-gate kernels) or QSIM_HIP_TRAJ_TILED. */
 int qsim_traj_run(void* batch, unsigned threads, int use_graph);
 int qsim_traj_zz(void* batch, unsigned t, double* zz, unsigned max,
                  unsigned* num);

@@ -7,7 +7,9 @@
 // each channel's Kraus operators against their probability bounds, the same
 // deferral of the chosen operators.  Runs of deferred gates are fused with
 // qsim's MultiQubitGateFuser (FP64 products) into the trajectory's operation
-// stream.  Where the bounds do not decide (an amplitude-damping draw past
+// stream -- or, for the tiled kernel, partitioned into tile passes by the
+// blocked-pass planner (qsim_amd/block_planner.h): a pass header followed by
+// its <= 4-qubit gates.  Where the bounds do not decide (an amplitude-damping draw past
 // 1 - gamma) the stream gets a decision operation that the device takes from
 // the state itself.  All streams then run in one launch
 // (qsim_hip_traj_run).
@@ -35,6 +37,7 @@
 #include "circuit_qsim_parser.h"
 #include "fuser_mqubit.h"
 #include "io.h"
+#include "qsim_amd/block_planner.h"
 #include "qsim_hip.h"
 #include "qsim_hip_driver.h"
 
@@ -140,9 +143,64 @@ Gate<double> Timed(const Gate<double>& g, unsigned time) {
   return h;
 }
 
+// How a trajectory's runs of deferred gates become operations: fused gates
+// of at most max_fused qubits, or (tile_bits >= 9) tile passes of
+// tile_bits qubits, low_bits of them the lowest.
+struct FusionSpec {
+  unsigned max_fused = 4;
+  unsigned tile_bits = 0;
+  unsigned low_bits = 5;
+};
+
+// Tile passes for a run of gates; false if the state is too small for them.
+bool FlushTiled(const std::vector<Operation<double>>& deferred, unsigned n,
+                const FusionSpec& spec, TrajPlan& plan) {
+  hip::BlockPlannerParameter bp;
+  bp.tile_bits = spec.tile_bits;
+  bp.low_bits = spec.low_bits;
+  hip::BlockPlan blocks;
+  if (!hip::PlanBlocks<IO>(bp, n, deferred, blocks)) return false;
+  for (const auto& step : blocks.steps) {
+    if (!step.blocked) return false;  // only gates are deferred
+  }
+  for (const auto& pass : blocks.blocks) {
+    qsim_hip_traj_op h{};
+    h.kind = QSIM_HIP_TRAJ_TILE_PASS;
+    h.k = pass.ks.size();
+    uint64_t mask = 0;
+    for (unsigned q : pass.tile) mask |= uint64_t{1} << q;
+    h.q[0] = uint32_t(mask);
+    h.q[1] = uint32_t(mask >> 32);
+    plan.ops.push_back(h);
+    plan.mats.emplace_back();  // no matrix of its own
+    size_t mo = 0;
+    for (size_t j = 0; j < pass.ks.size(); ++j) {
+      qsim_hip_traj_op o{};
+      o.kind = QSIM_HIP_TRAJ_GATE;
+      o.k = pass.ks[j];
+      for (unsigned b = 0; b < o.k; ++b) o.q[b] = pass.qs[4 * j + b];
+      const size_t nf = size_t{2} << (2 * o.k);
+      plan.ops.push_back(o);
+      plan.mats.emplace_back(pass.m.begin() + mo, pass.m.begin() + mo + nf);
+      mo += nf;
+    }
+  }
+  return true;
+}
+
 void Flush(std::vector<Operation<double>>& deferred, unsigned n,
-           unsigned max_fused, TrajPlan& plan) {
+           const FusionSpec& spec, TrajPlan& plan) {
   if (deferred.empty()) return;
+  if (spec.tile_bits >= 9) {
+    const size_t nops = plan.ops.size();
+    if (FlushTiled(deferred, n, spec, plan)) {
+      deferred.clear();
+      return;
+    }
+    plan.ops.resize(nops);
+    plan.mats.resize(nops);
+  }
+  const unsigned max_fused = spec.max_fused;
   MultiQubitGateFuser<IO>::Parameter fp;
   fp.max_fused_size = max_fused;
   fp.verbosity = 0;
@@ -161,8 +219,8 @@ void Flush(std::vector<Operation<double>>& deferred, unsigned n,
 }
 
 // RunIteration's sampling (lib/qtrajectory.h:270-356) for one trajectory.
-void PlanTrajectory(const NoisyCircuit& nc, uint64_t rep, unsigned max_fused,
-                    TrajPlan& plan) {
+void PlanTrajectory(const NoisyCircuit& nc, uint64_t rep,
+                    const FusionSpec& spec, TrajPlan& plan) {
   const unsigned n = nc.num_qubits;
   std::mt19937 rgen(rep);
   std::uniform_real_distribution<double> distr(0.0, 1.0);
@@ -197,7 +255,7 @@ void PlanTrajectory(const NoisyCircuit& nc, uint64_t rep, unsigned max_fused,
     }
     // The bounds do not decide: the state does (two-operator damping).  A
     // non-unitary history is normalised first, as NormalizeState does.
-    Flush(deferred, n, max_fused, plan);
+    Flush(deferred, n, spec, plan);
     qsim_hip_traj_op o{};
     o.kind = QSIM_HIP_TRAJ_DAMP_DECIDE;
     o.q[0] = ch.qubits[0];
@@ -210,7 +268,7 @@ void PlanTrajectory(const NoisyCircuit& nc, uint64_t rep, unsigned max_fused,
     plan.kraus.push_back(-2 - int32_t(o.slot));
     unitary = false;
   }
-  Flush(deferred, n, max_fused, plan);
+  Flush(deferred, n, spec, plan);
   plan.normalize_last = !unitary;
 }
 
@@ -249,12 +307,15 @@ unsigned qsim_traj_num_channels(void* batch, unsigned circuit) {
   return circuit < b->circuits.size() ? b->circuits[circuit].num_channels : 0;
 }
 
-int qsim_traj_plan(void* batch, const unsigned* circuit_of,
-                   const uint64_t* reps, unsigned num_traj,
-                   unsigned max_fused_size) {
-  auto* b = static_cast<TrajBatchHost*>(batch);
+}  // extern "C"
+
+namespace {
+
+int PlanBatch(TrajBatchHost* b, const unsigned* circuit_of,
+              const uint64_t* reps, unsigned num_traj,
+              const FusionSpec& spec) {
   if (b == nullptr || circuit_of == nullptr || reps == nullptr ||
-      num_traj == 0 || max_fused_size < 1 || max_fused_size > 4) {
+      num_traj == 0) {
     return QSIM_HIP_ERR_ARG;
   }
   for (unsigned t = 0; t < num_traj; ++t) {
@@ -270,8 +331,7 @@ int qsim_traj_plan(void* batch, const unsigned* circuit_of,
   for (int t = 0; t < int(num_traj); ++t) {
     b->plans[t].circuit = circuit_of[t];
     b->plans[t].rep = reps[t];
-    PlanTrajectory(b->circuits[circuit_of[t]], reps[t], max_fused_size,
-                   b->plans[t]);
+    PlanTrajectory(b->circuits[circuit_of[t]], reps[t], spec, b->plans[t]);
   }
   // Device order: largest states first.
   b->order.resize(num_traj);
@@ -318,6 +378,8 @@ int qsim_traj_plan(void* batch, const unsigned* circuit_of,
           pool.insert(pool.end(), m.begin(), m.end());
         }
         o.moff = it->second;
+      } else if (o.kind == QSIM_HIP_TRAJ_TILE_PASS) {
+        o.moff = 0;
       } else {
         o.moff = 0;
         o.moff2 = 8;
@@ -335,6 +397,34 @@ int qsim_traj_plan(void* batch, const unsigned* circuit_of,
   return qsim_hip_traj_prepare(descs.data(), descs.size(), ops.data(),
                                ops.size(), pool.data(), pool.size(),
                                pairs.data(), npairs, ndec, &b->handle);
+}
+
+}  // namespace
+
+extern "C" {
+
+int qsim_traj_plan(void* batch, const unsigned* circuit_of,
+                   const uint64_t* reps, unsigned num_traj,
+                   unsigned max_fused_size) {
+  if (max_fused_size < 1 || max_fused_size > 4) return QSIM_HIP_ERR_ARG;
+  FusionSpec spec;
+  spec.max_fused = max_fused_size;
+  return PlanBatch(static_cast<TrajBatchHost*>(batch), circuit_of, reps,
+                   num_traj, spec);
+}
+
+int qsim_traj_plan_tiled(void* batch, const unsigned* circuit_of,
+                         const uint64_t* reps, unsigned num_traj,
+                         unsigned tile_bits, unsigned low_bits) {
+  if (tile_bits < 9 || tile_bits > 14 || low_bits < 3 || low_bits > 9 ||
+      low_bits > tile_bits) {
+    return QSIM_HIP_ERR_ARG;
+  }
+  FusionSpec spec;
+  spec.tile_bits = tile_bits;
+  spec.low_bits = low_bits;
+  return PlanBatch(static_cast<TrajBatchHost*>(batch), circuit_of, reps,
+                   num_traj, spec);
 }
 
 int qsim_traj_run(void* batch, unsigned threads, int use_graph) {

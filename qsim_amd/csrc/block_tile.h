@@ -1,0 +1,261 @@
+// block_tile.h -- the tile machinery of blocked passes, shared by the
+// circuit kernel (block_kernels.hip) and the batched trajectory kernel
+// (traj_kernels.hip): kernel shapes, the op-record layout, the split-BF16
+// MFMA product of one 4-qubit op over an LDS tile, and tile staging between
+// HBM and LDS.
+//
+// Each op is a 4-qubit gate on tile-local positions (smaller gates are
+// padded with identity factors on the host) applied as the split-BF16 MFMA
+// product of the gate kernels (mfma_kernels.hip): the 16 x 16 complex
+// matrix becomes a real 32 x 32 A operand, 32 groups side by side the B
+// operand of v_mfma_f32_32x32x16_bf16; the matrix is split in three bf16
+// parts, the amplitudes in two, and Al Bh + Am Bh + Ah Bl + Ah Bh is
+// accumulated in FP32.  Operand maps (verified on gfx950):
+// a[e] = A[l%32][8(l/32) + e], b[e] = B[8(l/32) + e][l%32],
+// C register r of lane l = C[8(r/4) + 4(l/32) + r%4][l%32].
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "internal.h"
+#include "tile.h"
+
+namespace qhip {
+namespace tileop {
+
+typedef float v16f __attribute__((ext_vector_type(16)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+typedef float v2f __attribute__((ext_vector_type(2)));
+
+// Kernel shapes by tile qubits TB: up to 13 (64 KiB tiles), two 256-thread
+// workgroups per CU; 14 (128 KiB), one 512-thread workgroup per CU.  Both
+// give every thread 32 tile elements and every wave 256 registers.
+template <int TB>
+struct BlockShape {
+  static_assert(TB == 13 || TB == 14, "tile shape");
+  static constexpr int NT = TB == 13 ? 256 : 512;
+  static constexpr int kWaves = NT / 64;
+  static constexpr size_t kTileBytes = (size_t{1} << TB) * sizeof(float2);
+};
+// BF16 fragment elements of one 4-qubit op: 2 K blocks x (hi, mid, lo) x 512.
+constexpr size_t kOpFrag = 2 * 3 * 512;
+// One op record in 32-bit words: the fragments (1536 words); the member
+// tables -- LDS byte offsets, relative to a group's slot, of the members a
+// lane reads (8) and writes (8), for lane half 0 then 1 (32 words); the
+// group table -- LDS byte offset of lane j's group within a batch, j < 32
+// (32 words); padded to 8 KiB.
+constexpr int kFragWords = int(kOpFrag / 2);
+constexpr int kTableWords = 32;
+constexpr int kLaneWords = 32;
+constexpr int kOpWords = 2048;
+static_assert(kFragWords + kTableWords + kLaneWords <= kOpWords, "op record");
+template <int TB>
+constexpr size_t BlockLds() {
+  return BlockShape<TB>::kTileBytes + 2 * kOpWords * sizeof(uint32_t);
+}
+
+// The tile at LDS byte offset `b`: the kernel has no static LDS, so its
+// dynamic block starts at offset 0 (checked on the host before launch), and
+// an address is the offset itself, with no base to add.
+typedef __attribute__((address_space(3))) v2f LdsV2f;
+__device__ __forceinline__ LdsV2f* LdsAt(unsigned b) {
+  return reinterpret_cast<LdsV2f*>(size_t{b});
+}
+
+// One 4-qubit op over the LDS tile.  A group (the 16 amplitudes the op
+// mixes) is fixed by its bits at the tile's non-target positions; the host
+// splits those positions into 5 lane positions (chosen so that the 32
+// groups of a batch fall into distinct LDS banks) and tb - 9 batch
+// positions `bp`.  Wave `wave` (wave-uniform) of `nwaves` takes batches
+// bt = wave, wave + nwaves, ...; lanes j and j + 32 serve the batch's j-th
+// group.  `f` holds the op's fragments (hi, mid, lo per K block); rd / wr
+// are this lane's member offsets with its group's offset already folded in
+// (LDS bytes).  TileSwz is linear over GF(2) and the lane part, batch part
+// and member offsets occupy disjoint bits, so a member's slot is the XOR of
+// the three parts' swizzled slots: the batch part is scalar work and an
+// address is one XOR (the tile sits at LDS offset 0).
+__device__ __forceinline__ unsigned BatchSlot(unsigned bt,
+                                              const unsigned (&b)[5]) {
+  unsigned pb = 0;
+#pragma unroll
+  for (int i = 0; i < 5; ++i) pb |= ((bt >> i) & 1u) << b[i];
+  return TileSwz(pb) * unsigned(sizeof(float2));
+}
+
+// Lane half h's 8 members of its group as bf16 (hi, lo) pairs.
+__device__ __forceinline__ void SplitBf16(const v2f (&v)[8], v8bf (&bh)[2],
+                                          v8bf (&bl)[2]) {
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const v2bf hi = __builtin_convertvector(v[q], v2bf);
+    const v2f r = v[q] - __builtin_convertvector(hi, v2f);
+    const v2bf lo = __builtin_convertvector(r, v2bf);
+    bh[q >> 2][2 * (q & 3)] = hi[0];
+    bh[q >> 2][2 * (q & 3) + 1] = hi[1];
+    bl[q >> 2][2 * (q & 3)] = lo[0];
+    bl[q >> 2][2 * (q & 3) + 1] = lo[1];
+  }
+}
+
+// The op on one batch: Al Bh + Am Bh + Ah Bl + Ah Bh per K block, one
+// accumulator chain.
+__device__ __forceinline__ v16f MfmaChain(const v8bf (&f)[6],
+                                          const v8bf (&bh)[2],
+                                          const v8bf (&bl)[2]) {
+  v16f c = v16f{};
+  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[2], bh[0], c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[1], bh[0], c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[0], bl[0], c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[0], bh[0], c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[5], bh[1], c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[4], bh[1], c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[3], bl[1], c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[3], bh[1], c, 0, 0, 0);
+  return c;
+}
+
+// Lane l holds (re, im) of members 4 s + 2 h + {0, 1}.
+__device__ __forceinline__ void WriteBatch(unsigned sp, const uint32_t (&wr)[8],
+                                           const v16f& c) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      *LdsAt(sp ^ wr[2 * q + u]) = v2f{c[4 * q + 2 * u], c[4 * q + 2 * u + 1]};
+    }
+}
+
+__device__ __forceinline__ void ApplyTileOp(const uint8_t* bp,
+                                            const v8bf (&f)[6],
+                                            const uint32_t (&rd)[8],
+                                            const uint32_t (&wr)[8],
+                                            unsigned nbatch, unsigned wave,
+                                            unsigned nwaves) {
+  const unsigned b[5] = {bp[0], bp[1], bp[2], bp[3], bp[4]};
+  // Two batches at a time (their groups are disjoint): batch 1's reads and
+  // split overlap batch 0's MFMA chain.
+  unsigned bt = wave;
+  for (; bt + nwaves < nbatch; bt += 2 * nwaves) {
+    const unsigned sp0 = BatchSlot(bt, b);
+    const unsigned sp1 = BatchSlot(bt + nwaves, b);
+    v2f v0[8], v1[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v0[q] = *LdsAt(sp0 ^ rd[q]);
+    v8bf bh0[2], bl0[2], bh1[2], bl1[2];
+    SplitBf16(v0, bh0, bl0);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v1[q] = *LdsAt(sp1 ^ rd[q]);
+    const v16f c0 = MfmaChain(f, bh0, bl0);
+    SplitBf16(v1, bh1, bl1);
+    const v16f c1 = MfmaChain(f, bh1, bl1);
+    WriteBatch(sp0, wr, c0);
+    WriteBatch(sp1, wr, c1);
+  }
+  if (bt < nbatch) {
+    const unsigned sp0 = BatchSlot(bt, b);
+    v2f v0[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v0[q] = *LdsAt(sp0 ^ rd[q]);
+    v8bf bh0[2], bl0[2];
+    SplitBf16(v0, bh0, bl0);
+    WriteBatch(sp0, wr, MfmaChain(f, bh0, bl0));
+  }
+}
+
+// Offset (in amplitudes) of tile-local high index x: x's bits deposited at
+// the tile's high qubits.
+__device__ __forceinline__ uint64_t DepositHigh(unsigned x, const BlockGeom& g) {
+  uint64_t o = 0;
+  for (unsigned b = 0; b < g.h; ++b) o += ((x >> b) & 1u) ? g.hbit[b] : 0;
+  return o;
+}
+
+// Tile staging with addresses split into a per-thread part and a per-element
+// part.  Element i of thread t is tile-local amplitude a = t + NT i (valid
+// for i < 2^tb / NT, a wave-uniform test).  The host keeps the low run
+// within the thread bits (Lc <= log2 NT), so a's low bits come from t alone
+// and its high index is (t >> Lc) | (i << (log2 NT - Lc)) with disjoint
+// bits: its offset is D(t >> Lc) + D(i << (log2 NT - Lc)), one add per
+// element instead of a deposit loop.  The element parts are wave-uniform
+// and kept in units of 2^Lc amplitudes (32 bits: no tile qubit is 32 above
+// the low run).
+template <int PER, int NT>
+struct TileIo {
+  static_assert(NT == 256 || NT == 512, "thread bits");
+  static constexpr unsigned kLogNT = NT == 256 ? 8 : 9;
+  uint64_t tofs;       // this thread's part
+  uint32_t eofs[PER];  // the element parts >> Lc
+
+  // LDS slot of element i: TileSwz is linear and changes only the low four
+  // bits, so TileSwz(t + NT i) = (TileSwz(t) ^ (TileSwz(NT i) & 15)) + NT i
+  // with TileSwz(t) < NT: the NT i part is an instruction offset and the
+  // rest takes one XOR with a constant.
+  __device__ __forceinline__ unsigned Slot(int i) const {
+    return (TileSwz(threadIdx.x) ^ (TileSwz(NT * unsigned(i)) & 15u)) +
+           NT * unsigned(i);
+  }
+
+  __device__ __forceinline__ void Init(const BlockGeom& g) {
+    const unsigned t = threadIdx.x;
+    tofs = (t & ((1u << g.Lc) - 1)) + DepositHigh(t >> g.Lc, g);
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      eofs[i] = uint32_t(DepositHigh(unsigned(i) << (kLogNT - g.Lc), g) >> g.Lc);
+    }
+  }
+
+  // Issues the loads of the tile at `base` into v (this thread's elements),
+  // those of the quarters of v whose bits are set in MASK (quarter c holds
+  // elements c PER/4 .. (c+1) PER/4 - 1).
+  template <bool FULL, unsigned MASK>
+  __device__ __forceinline__ void Fetch(const float2* __restrict__ s,
+                                        uint64_t base, const BlockGeom& g,
+                                        float2 (&v)[PER]) const {
+    const unsigned nper = (1u << g.tb) >> kLogNT;  // elements per thread
+    const float2* p = s + base + tofs;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      if (((MASK >> (4 * i / PER)) & 1u) && (FULL || unsigned(i) < nper)) {
+        v[i] = p[uint64_t(eofs[i]) << g.Lc];
+      }
+    }
+  }
+
+  // Writes fetched elements to their LDS slots.
+  template <bool FULL>
+  __device__ __forceinline__ void Put(const BlockGeom& g,
+                                      const float2 (&v)[PER],
+                                      float2* lds) const {
+    const unsigned nper = (1u << g.tb) >> kLogNT;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      if (FULL || unsigned(i) < nper) lds[Slot(i)] = v[i];
+    }
+  }
+
+  template <int CH, bool FULL>
+  __device__ __forceinline__ void Store(float2* __restrict__ s, uint64_t base,
+                                        const BlockGeom& g,
+                                        const float2* lds) const {
+    const unsigned nper = (1u << g.tb) >> kLogNT;
+    float2* p = s + base + tofs;
+#pragma unroll
+    for (int i0 = 0; i0 < PER; i0 += CH) {
+      float2 v[CH];
+#pragma unroll
+      for (int i = 0; i < CH; ++i) {
+        if (FULL || unsigned(i0 + i) < nper) v[i] = lds[Slot(i0 + i)];
+      }
+#pragma unroll
+      for (int i = 0; i < CH; ++i) {
+        if (FULL || unsigned(i0 + i) < nper) p[uint64_t(eofs[i0 + i]) << g.Lc] = v[i];
+      }
+    }
+  }
+};
+
+}  // namespace tileop
+}  // namespace qhip

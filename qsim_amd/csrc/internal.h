@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <mutex>
+#include <vector>
 
 #include "qsim_hip.h"
 
@@ -197,6 +198,14 @@ void BuildFragBf16(int K, const double* m, uint16_t* frag);
 int LaunchGateBlock(float2* s, unsigned n, const unsigned* tile, unsigned nt,
                     unsigned nops, const unsigned* ks, const unsigned* qs,
                     const double* m64, const size_t* moff);
+
+// The host half of a blocked pass: validates the program and builds its
+// geometry, per-op tables and op records (kOpWords 32-bit words each, in
+// `fb` as 16-bit halves).
+int PrepareGateBlock(unsigned n, const unsigned* tile, unsigned nt,
+                     unsigned nops, const unsigned* ks, const unsigned* qs,
+                     const double* m64, const size_t* moff, BlockGeom& g,
+                     BlockProgram& p, std::vector<uint16_t>& fb);
 
 // The same pass prepared once (device fragments kept) and run many times.
 int PrepareBlockHandle(unsigned n, const unsigned* tile, unsigned nt,

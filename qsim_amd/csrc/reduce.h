@@ -14,10 +14,10 @@ __device__ inline double WaveSum(double v) {
 }
 
 // Sum over a workgroup of BS threads; the result is valid in thread 0.
+// `part` is LDS scratch of BS / 64 doubles.
 template <int BS>
-__device__ inline double BlockSum(double v) {
+__device__ inline double BlockSum(double v, double* part) {
   static_assert(BS % 64 == 0 && BS <= 1024, "block size");
-  __shared__ double part[BS / 64];
   v = WaveSum(v);
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   if (l == 0) part[w] = v;
@@ -28,6 +28,12 @@ __device__ inline double BlockSum(double v) {
   }
   __syncthreads();
   return v;
+}
+
+template <int BS>
+__device__ inline double BlockSum(double v) {
+  __shared__ double part[BS / 64];
+  return BlockSum<BS>(v, part);
 }
 
 template <int BS>

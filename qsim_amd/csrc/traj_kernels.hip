@@ -28,8 +28,13 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <string>
+#include <type_traits>
+#include <unordered_map>
 #include <vector>
 
+#include "block_tile.h"
 #include "internal.h"
 #include "reduce.h"
 
@@ -37,7 +42,15 @@ namespace qhip {
 
 namespace {
 
+using namespace tileop;
+
+// Pairs per <Z_a Z_b> sweep (FP64 accumulators per thread: 32 would not fit
+// beside the tile kernel's registers) and amplitudes per thread per step of
+// a whole-state sweep (64 KiB of loads in flight per workgroup).
+template <int NT>
 constexpr unsigned kPairChunk = 16;
+template <int NT>
+constexpr unsigned kSweep = 8192 / NT;
 
 template <int NT, int K>
 __device__ void ApplyGateK(float2* __restrict__ s, unsigned n,
@@ -82,127 +95,397 @@ __device__ void ApplyGateK(float2* __restrict__ s, unsigned n,
   }
 }
 
+// LDS scratch of the reductions: NT / 64 wave partials, then two broadcast
+// slots.
+template <int NT>
+constexpr unsigned kRedWords = NT / 64 + 2;
+
+template <int NT>
+__device__ void ZeroState(float2* __restrict__ s, uint64_t N) {
+  for (uint64_t i = threadIdx.x; i < N; i += NT) {
+    s[i] = float2{i == 0 ? 1.0f : 0.0f, 0.0f};
+  }
+}
+
+// A fused gate: its FP32 matrix staged in LDS (lm), then one sweep.
+template <int NT>
+__device__ void TrajGate(float2* __restrict__ s, unsigned n,
+                         const qsim_hip_traj_op& o,
+                         const float* __restrict__ mats, float* lm) {
+  const unsigned D = 1u << o.k;
+  for (unsigned i = threadIdx.x; i < 2 * D * D; i += NT) lm[i] = mats[o.moff + i];
+  __syncthreads();
+  switch (o.k) {
+    case 1: ApplyGateK<NT, 1>(s, n, o, lm); break;
+    case 2: ApplyGateK<NT, 2>(s, n, o, lm); break;
+    case 3: ApplyGateK<NT, 3>(s, n, o, lm); break;
+    default: ApplyGateK<NT, 4>(s, n, o, lm); break;
+  }
+  __syncthreads();
+}
+
+// Amplitude-damping decision on qubit q: FP64 populations of |0> and |1>,
+// the choice of K0 or K1, the chosen operator applied.
+template <int NT>
+__device__ void TrajDecide(float2* __restrict__ s, unsigned n,
+                           const qsim_hip_traj_op& o,
+                           const float* __restrict__ mats,
+                           int32_t* __restrict__ dec_out, double* red) {
+  const uint64_t N = uint64_t{1} << n;
+  const unsigned q = o.q[0];
+  constexpr unsigned kS = kSweep<NT>;
+  double p0 = 0.0, p1 = 0.0;
+  for (uint64_t i0 = threadIdx.x; i0 < N; i0 += kS * NT) {
+    float2 a[kS];
+#pragma unroll
+    for (unsigned u = 0; u < kS; ++u) {
+      const uint64_t i = i0 + u * NT;
+      a[u] = i < N ? s[i] : float2{0.0f, 0.0f};
+    }
+#pragma unroll
+    for (unsigned u = 0; u < kS; ++u) {
+      const double w = double(a[u].x) * a[u].x + double(a[u].y) * a[u].y;
+      if (((i0 + u * NT) >> q) & 1) p1 += w; else p0 += w;
+    }
+  }
+  p0 = BlockSum<NT>(p0, red);
+  p1 = BlockSum<NT>(p1, red);
+  if (threadIdx.x == 0) {
+    red[NT / 64] = p0;
+    red[NT / 64 + 1] = p1;
+  }
+  __syncthreads();
+  p0 = red[NT / 64];
+  p1 = red[NT / 64 + 1];
+  __syncthreads();
+  // RunIteration normalises a non-unitary state first, then samples K0 iff
+  // r < <psi|K0^dagger K0|psi> (the bound 1 - gamma cancels out of cp).
+  const double norm = p0 + p1;
+  const double prob0 = (p0 + double(o.kdk1) * p1) / norm;
+  const int pick = double(o.r) < prob0 ? 0 : 1;
+  const float scale = o.pad & 1u ? float(1.0 / std::sqrt(norm)) : 1.0f;
+  const float* km = mats + (pick == 0 ? o.moff : o.moff2);
+  const float m00r = km[0] * scale, m00i = km[1] * scale;
+  const float m01r = km[2] * scale, m01i = km[3] * scale;
+  const float m10r = km[4] * scale, m10i = km[5] * scale;
+  const float m11r = km[6] * scale, m11i = km[7] * scale;
+  const uint64_t half = N >> 1, qb = uint64_t{1} << q;
+  constexpr unsigned kPairs = kS / 2;
+  for (uint64_t g0 = threadIdx.x; g0 < half; g0 += kPairs * NT) {
+    float2 a[kPairs], b[kPairs];
+    uint64_t i0[kPairs];
+#pragma unroll
+    for (unsigned u = 0; u < kPairs; ++u) {
+      const uint64_t g = g0 + u * NT;
+      i0[u] = ((g >> q) << (q + 1)) | (g & (qb - 1));
+      if (g < half) {
+        a[u] = s[i0[u]];
+        b[u] = s[i0[u] + qb];
+      }
+    }
+#pragma unroll
+    for (unsigned u = 0; u < kPairs; ++u) {
+      if (g0 + u * NT < half) {
+        s[i0[u]] = float2{m00r * a[u].x - m00i * a[u].y + m01r * b[u].x - m01i * b[u].y,
+                          m00r * a[u].y + m00i * a[u].x + m01r * b[u].y + m01i * b[u].x};
+        s[i0[u] + qb] = float2{m10r * a[u].x - m10i * a[u].y + m11r * b[u].x - m11i * b[u].y,
+                               m10r * a[u].y + m10i * a[u].x + m11r * b[u].y + m11i * b[u].x};
+      }
+    }
+  }
+  if (threadIdx.x == 0) dec_out[o.slot] = pick;
+  __syncthreads();
+}
+
+// Signs (-1)^(bit a + bit b) of index i for the pairs (pa[j], pb[j]),
+// j < kPC, as bit j of a mask.  The map is linear over XOR, so the sign mask
+// of i = x | y (disjoint bits) is the XOR of those of x and y.
+template <unsigned kPC>
+__device__ __forceinline__ uint32_t PairSigns(uint64_t i,
+                                              const unsigned (&pa)[kPC],
+                                              const unsigned (&pb)[kPC]) {
+  uint32_t m = 0;
+#pragma unroll
+  for (unsigned j = 0; j < kPC; ++j) {
+    m |= uint32_t(((i >> pa[j]) ^ (i >> pb[j])) & 1u) << j;
+  }
+  return m;
+}
+
+// <Z_a Z_b> = sum_i |s_i|^2 (-1)^(bit a + bit b) / sum_i |s_i|^2 for the
+// trajectory's pairs, FP64-accumulated.  A non-unitary trajectory ends
+// normalised (lib/qtrajectory.h:377-383): its norm goes to norm_out and the
+// scaling is applied where the state is read (qsim_hip_traj_state), which
+// saves a read-write sweep of every state.
+template <int NT>
+__device__ void TrajFinish(const float2* __restrict__ s,
+                           const qsim_hip_traj_desc& d,
+                           const uint32_t* __restrict__ pairs,
+                           double* __restrict__ zz_out,
+                           double* __restrict__ norm_out, double* red) {
+  constexpr unsigned kS = kSweep<NT>;
+  constexpr unsigned kPC = kPairChunk<NT>;
+  const uint64_t N = uint64_t{1} << d.num_qubits;
+  double total = 0.0;  // the norm, in thread 0
+  bool have_norm = false;
+  for (unsigned p0 = 0; p0 < d.num_pairs; p0 += kPC) {
+    const unsigned np = std::min(kPC, d.num_pairs - p0);
+    unsigned pa[kPC], pb[kPC];
+    double acc[kPC];
+#pragma unroll
+    for (unsigned j = 0; j < kPC; ++j) {
+      pa[j] = j < np ? pairs[2 * (d.pair_offset + p0 + j)] : 0;
+      pb[j] = j < np ? pairs[2 * (d.pair_offset + p0 + j) + 1] : 0;
+      acc[j] = 0.0;
+    }
+    // Element u of a step is i0 + u NT = i0 | u NT (i0's bits from
+    // log2 NT to log2 (kS NT) are zero): per-element sign masks are
+    // uniform, per-step ones one per thread.
+    uint32_t su[kS];
+#pragma unroll
+    for (unsigned u = 0; u < kS; ++u) su[u] = PairSigns(uint64_t(u) * NT, pa, pb);
+    double nrm = 0.0;
+    for (uint64_t i0 = threadIdx.x; i0 < N; i0 += kS * NT) {
+      const uint32_t s0 = PairSigns(i0, pa, pb);
+      float2 a[kS];
+#pragma unroll
+      for (unsigned u = 0; u < kS; ++u) {
+        const uint64_t i = i0 + u * NT;
+        a[u] = i < N ? s[i] : float2{0.0f, 0.0f};
+      }
+#pragma unroll
+      for (unsigned u = 0; u < kS; ++u) {
+        const double w = double(a[u].x) * a[u].x + double(a[u].y) * a[u].y;
+        const uint32_t m = s0 ^ su[u];
+        nrm += w;
+#pragma unroll
+        for (unsigned j = 0; j < kPC; ++j) acc[j] += (m >> j) & 1u ? -w : w;
+      }
+    }
+    nrm = BlockSum<NT>(nrm, red);
+    total = nrm;
+    have_norm = true;
+    // Unrolled, so that acc stays in registers.
+#pragma unroll
+    for (unsigned j = 0; j < kPC; ++j) {
+      const double v = BlockSum<NT>(acc[j], red);
+      if (threadIdx.x == 0 && j < np) zz_out[d.pair_offset + p0 + j] = v / nrm;
+    }
+  }
+  if (!(d.pad & 1u)) return;
+  if (!have_norm) {
+    double nrm = 0.0;
+    for (uint64_t i0 = threadIdx.x; i0 < N; i0 += kS * NT) {
+      float2 a[kS];
+#pragma unroll
+      for (unsigned u = 0; u < kS; ++u) {
+        const uint64_t i = i0 + u * NT;
+        a[u] = i < N ? s[i] : float2{0.0f, 0.0f};
+      }
+#pragma unroll
+      for (unsigned u = 0; u < kS; ++u) {
+        nrm += double(a[u].x) * a[u].x + double(a[u].y) * a[u].y;
+      }
+    }
+    total = BlockSum<NT>(nrm, red);
+  }
+  if (threadIdx.x == 0) norm_out[blockIdx.x] = total;
+}
+
+// Workgroup t runs trajectory t, one sweep of the state per operation.  A
+// tile-pass header is skipped: the gates of its pass follow it in the
+// stream and run here one sweep each.
 template <int NT>
 __global__ void __launch_bounds__(NT)
 TrajectoryKernel(const qsim_hip_traj_desc* __restrict__ descs,
                  const qsim_hip_traj_op* __restrict__ ops,
                  const float* __restrict__ mats,
                  const uint32_t* __restrict__ pairs, float2* __restrict__ pool,
-                 double* __restrict__ zz_out, int32_t* __restrict__ dec_out) {
+                 double* __restrict__ zz_out, double* __restrict__ norm_out,
+                 int32_t* __restrict__ dec_out) {
   __shared__ float lm[2 * 16 * 16];
-  __shared__ double shared_d[2];
+  __shared__ double red[kRedWords<NT>];
   const qsim_hip_traj_desc d = descs[blockIdx.x];
   const unsigned n = d.num_qubits;
-  const uint64_t N = uint64_t{1} << n;
   float2* s = pool + d.state_offset;
-
-  for (uint64_t i = threadIdx.x; i < N; i += NT) {
-    s[i] = float2{i == 0 ? 1.0f : 0.0f, 0.0f};
-  }
+  ZeroState<NT>(s, uint64_t{1} << n);
   __syncthreads();
-
   for (uint64_t oi = d.op_begin; oi < d.op_end; ++oi) {
     const qsim_hip_traj_op o = ops[oi];
     if (o.kind == QSIM_HIP_TRAJ_GATE) {
-      const unsigned D = 1u << o.k;
-      for (unsigned i = threadIdx.x; i < 2 * D * D; i += NT) lm[i] = mats[o.moff + i];
-      __syncthreads();
-      switch (o.k) {
-        case 1: ApplyGateK<NT, 1>(s, n, o, lm); break;
-        case 2: ApplyGateK<NT, 2>(s, n, o, lm); break;
-        case 3: ApplyGateK<NT, 3>(s, n, o, lm); break;
-        default: ApplyGateK<NT, 4>(s, n, o, lm); break;
-      }
-      __syncthreads();
-      continue;
+      TrajGate<NT>(s, n, o, mats, lm);
+    } else if (o.kind == QSIM_HIP_TRAJ_DAMP_DECIDE) {
+      TrajDecide<NT>(s, n, o, mats, dec_out, red);
     }
-    // Amplitude-damping decision on qubit q: FP64 populations of |0> and |1>.
-    const unsigned q = o.q[0];
-    double p0 = 0.0, p1 = 0.0;
-    for (uint64_t i = threadIdx.x; i < N; i += NT) {
-      const float2 a = s[i];
-      const double w = double(a.x) * a.x + double(a.y) * a.y;
-      if ((i >> q) & 1) p1 += w; else p0 += w;
-    }
-    p0 = BlockSum<NT>(p0);
-    p1 = BlockSum<NT>(p1);
-    if (threadIdx.x == 0) {
-      shared_d[0] = p0;
-      shared_d[1] = p1;
-    }
-    __syncthreads();
-    p0 = shared_d[0];
-    p1 = shared_d[1];
-    __syncthreads();
-    // RunIteration normalises a non-unitary state first, then samples K0 iff
-    // r < <psi|K0^dagger K0|psi> (the bound 1 - gamma cancels out of cp).
-    const double norm = p0 + p1;
-    const double prob0 = (p0 + double(o.kdk1) * p1) / norm;
-    const int pick = double(o.r) < prob0 ? 0 : 1;
-    const float scale = o.pad & 1u ? float(1.0 / std::sqrt(norm)) : 1.0f;
-    const float* km = mats + (pick == 0 ? o.moff : o.moff2);
-    const float m00r = km[0] * scale, m00i = km[1] * scale;
-    const float m01r = km[2] * scale, m01i = km[3] * scale;
-    const float m10r = km[4] * scale, m10i = km[5] * scale;
-    const float m11r = km[6] * scale, m11i = km[7] * scale;
-    const uint64_t half = N >> 1, qb = uint64_t{1} << q;
-    for (uint64_t g = threadIdx.x; g < half; g += NT) {
-      const uint64_t i0 = ((g >> q) << (q + 1)) | (g & (qb - 1));
-      const float2 a = s[i0], b = s[i0 + qb];
-      s[i0] = float2{m00r * a.x - m00i * a.y + m01r * b.x - m01i * b.y,
-                     m00r * a.y + m00i * a.x + m01r * b.y + m01i * b.x};
-      s[i0 + qb] = float2{m10r * a.x - m10i * a.y + m11r * b.x - m11i * b.y,
-                          m10r * a.y + m10i * a.x + m11r * b.y + m11i * b.x};
-    }
-    if (threadIdx.x == 0) dec_out[o.slot] = pick;
-    __syncthreads();
   }
+  TrajFinish<NT>(s, d, pairs, zz_out, norm_out, red);
+}
 
-  // <Z_a Z_b> = sum_i |s_i|^2 (-1)^(bit a + bit b) / sum_i |s_i|^2.
-  for (unsigned p0 = 0; p0 < d.num_pairs; p0 += kPairChunk) {
-    const unsigned np = std::min(kPairChunk, d.num_pairs - p0);
-    unsigned pa[kPairChunk], pb[kPairChunk];
-    double acc[kPairChunk];
-#pragma unroll
-    for (unsigned j = 0; j < kPairChunk; ++j) {
-      pa[j] = j < np ? pairs[2 * (d.pair_offset + p0 + j)] : 0;
-      pb[j] = j < np ? pairs[2 * (d.pair_offset + p0 + j) + 1] : 0;
-      acc[j] = 0.0;
-    }
-    double nrm = 0.0;
-    for (uint64_t i = threadIdx.x; i < N; i += NT) {
-      const float2 a = s[i];
-      const double w = double(a.x) * a.x + double(a.y) * a.y;
-      nrm += w;
-#pragma unroll
-      for (unsigned j = 0; j < kPairChunk; ++j) {
-        const unsigned par = unsigned((i >> pa[j]) ^ (i >> pb[j])) & 1u;
-        acc[j] += par ? -w : w;
-      }
-    }
-    nrm = BlockSum<NT>(nrm);
-    for (unsigned j = 0; j < kPairChunk; ++j) {
-      const double v = BlockSum<NT>(acc[j]);
-      if (threadIdx.x == 0 && j < np) zz_out[d.pair_offset + p0 + j] = v / nrm;
-    }
-  }
+// A tile pass of a trajectory on the device: geometry, program, and the
+// index of its first op record (kOpWords words each) in the record pool.
+struct TrajTilePass {
+  BlockGeom g;
+  BlockProgram p;
+  uint64_t rec;
+};
 
-  // A non-unitary trajectory ends normalised (lib/qtrajectory.h:377-383).
-  if (d.pad & 1u) {
-    double nrm = 0.0;
-    for (uint64_t i = threadIdx.x; i < N; i += NT) {
-      const float2 a = s[i];
-      nrm += double(a.x) * a.x + double(a.y) * a.y;
+// The tiled kernel's shapes (block_tile.h): TB = 13, tiles of up to 13
+// qubits, 256 threads per trajectory, two workgroups per CU; TB = 14, tiles
+// of up to 14 qubits, 512 threads, one workgroup per CU (twice the bytes in
+// flight per trajectory, fewer passes, half as many trajectories at once).
+// One tile pass of the workgroup's trajectory: for every tile of the state,
+// stage it in LDS, run the pass's ops on it (split-BF16 MFMA), write it
+// back, with the next tile's loads in flight meanwhile.  The pass's op
+// records cycle through a two-slot ring in LDS as in the circuit kernel's
+// blocked passes (block_kernels.hip).  The tile sits at LDS offset 0 (no
+// static LDS in the kernel, checked on the host).  FULL: tiles of TB
+// qubits, whose next-tile loads go out in quarters over the first ops (each
+// quarter right after an op's record loads, so that in-order completion
+// never makes a record wait behind it); smaller tiles issue them all during
+// the last op.
+template <int TB, bool FULL>
+__device__ __forceinline__ void TrajTileRun(float2* __restrict__ s,
+                                            const TrajTilePass* __restrict__ tp,
+                                            const uint16_t* __restrict__ recs,
+                                            char* lds_raw) {
+  constexpr int NT = BlockShape<TB>::NT;
+  constexpr int PER = 32;
+  constexpr int kRingQuads = kOpWords / 4 / NT;
+  static_assert(kOpWords % (4 * NT) == 0 && (1 << TB) / NT == PER, "shape");
+  float2* lds = reinterpret_cast<float2*>(lds_raw);
+  uint32_t* ring =
+      reinterpret_cast<uint32_t*>(lds_raw + BlockShape<TB>::kTileBytes);
+  uint4* ring4 = reinterpret_cast<uint4*>(ring);
+  const BlockGeom& g = tp->g;
+  const BlockProgram& p = tp->p;
+  const unsigned nops = p.nops;
+  const int l = threadIdx.x & 63;
+  const unsigned wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const unsigned nbatch = (1u << g.tb) >> 9;
+  const uint4* gfrag =
+      reinterpret_cast<const uint4*>(recs) + tp->rec * (kOpWords / 4);
+  __syncthreads();  // the ring area was scratch until now
+#pragma unroll
+  for (int w = 0; w < kRingQuads; ++w) {
+    ring4[threadIdx.x + NT * w] = gfrag[threadIdx.x + NT * w];
+  }
+  TileIo<PER, NT> io;
+  io.Init(g);
+  unsigned k = 0;
+  auto run_op = [&](unsigned op, unsigned nxt, auto&& extra) {
+    const uint4* gn = gfrag + size_t(nxt) * (kOpWords / 4);
+    const uint4 pre0 = gn[threadIdx.x];
+    const uint4 pre1 = kRingQuads > 1 ? gn[threadIdx.x + NT] : uint4{};
+    extra();
+    const uint32_t* cur = ring + (k & 1u) * kOpWords;
+    const v8bf* fp = reinterpret_cast<const v8bf*>(cur);
+    v8bf f[6];
+#pragma unroll
+    for (int r = 0; r < 6; ++r) f[r] = fp[r * 64 + l];
+    const uint32_t* tab = cur + kFragWords + 16 * (l >> 5);
+    const uint32_t sj = cur[kFragWords + kTableWords + (l & 31)];
+    uint32_t rd[8], wr[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      rd[q] = tab[q] ^ sj;
+      wr[q] = tab[8 + q] ^ sj;
     }
-    nrm = BlockSum<NT>(nrm);
-    if (threadIdx.x == 0) shared_d[0] = nrm;
+    ApplyTileOp(p.bpos[op], f, rd, wr, nbatch, wave, BlockShape<TB>::kWaves);
+    uint4* slot = ring4 + ((k + 1) & 1u) * (kOpWords / 4);
+    slot[threadIdx.x] = pre0;
+    if (kRingQuads > 1) slot[threadIdx.x + NT] = pre1;
     __syncthreads();
-    const float a = float(1.0 / std::sqrt(shared_d[0]));
-    for (uint64_t i = threadIdx.x; i < N; i += NT) {
-      const float2 v = s[i];
-      s[i] = float2{v.x * a, v.y * a};
+    ++k;
+  };
+  float2 buf[PER];
+  io.template Fetch<FULL, 15u>(s, 0, g, buf);  // tile 0 starts at amplitude 0
+  io.template Put<FULL>(g, buf, lds);
+  __syncthreads();
+  for (uint64_t tile = 0; tile < g.ntiles; ++tile) {
+    const uint64_t base = InsertZeros(tile << g.Lc, g.outer, g.nouter);
+    // The next tile, or this one again after the last (an in-bounds read
+    // that is never used).
+    const uint64_t next = tile + 1 < g.ntiles ? tile + 1 : tile;
+    const uint64_t nbase = InsertZeros(next << g.Lc, g.outer, g.nouter);
+    auto fetch = [&](auto mask) {
+      return [&, mask] {
+        io.template Fetch<FULL, decltype(mask)::value>(s, nbase, g, buf);
+      };
+    };
+    using Q0 = std::integral_constant<unsigned, 1u>;
+    using Q1 = std::integral_constant<unsigned, 2u>;
+    using Q2 = std::integral_constant<unsigned, 4u>;
+    switch (FULL ? nops : 1u) {
+      case 1:
+        for (unsigned op = 0; op + 1 < nops; ++op) run_op(op, op + 1, [] {});
+        run_op(nops - 1, 0, fetch(std::integral_constant<unsigned, 15u>{}));
+        break;
+      case 2:
+        run_op(0, 1, fetch(Q0{}));
+        run_op(1, 0, fetch(std::integral_constant<unsigned, 14u>{}));
+        break;
+      case 3:
+        run_op(0, 1, fetch(Q0{}));
+        run_op(1, 2, fetch(Q1{}));
+        run_op(2, 0, fetch(std::integral_constant<unsigned, 12u>{}));
+        break;
+      default:
+        run_op(0, 1, fetch(Q0{}));
+        run_op(1, 2, fetch(Q1{}));
+        run_op(2, 3, fetch(Q2{}));
+        for (unsigned op = 3; op + 1 < nops; ++op) run_op(op, op + 1, [] {});
+        run_op(nops - 1, 0, fetch(std::integral_constant<unsigned, 8u>{}));
+        break;
+    }
+    io.template Store<16, FULL>(s, base, g, lds);
+    __syncthreads();
+    io.template Put<FULL>(g, buf, lds);
+    __syncthreads();
+  }
+}
+
+// Workgroup t runs trajectory t with its gate runs as tile passes: one
+// stream of the state per pass instead of one per fused gate.  Gates outside
+// a pass (states under 9 qubits) and damping decisions run as sweeps over
+// the state with the workgroup's threads; their scratch (the gate matrix,
+// the reduction partials) reuses the ring area of the LDS.
+template <int TB>
+__global__ void __launch_bounds__(BlockShape<TB>::NT)
+__attribute__((amdgpu_waves_per_eu(2)))
+TrajTileKernel(const qsim_hip_traj_desc* __restrict__ descs,
+               const qsim_hip_traj_op* __restrict__ ops,
+               const float* __restrict__ mats,
+               const TrajTilePass* __restrict__ passes,
+               const uint16_t* __restrict__ recs,
+               const uint32_t* __restrict__ pairs, float2* __restrict__ pool,
+               double* __restrict__ zz_out, double* __restrict__ norm_out,
+               int32_t* __restrict__ dec_out) {
+  constexpr int NT = BlockShape<TB>::NT;
+  extern __shared__ char lds_raw[];
+  char* aux = lds_raw + BlockShape<TB>::kTileBytes;
+  float* lm = reinterpret_cast<float*>(aux);
+  double* red = reinterpret_cast<double*>(aux + 2 * 16 * 16 * sizeof(float));
+  const qsim_hip_traj_desc d = descs[blockIdx.x];
+  const unsigned n = d.num_qubits;
+  float2* s = pool + d.state_offset;
+  ZeroState<NT>(s, uint64_t{1} << n);
+  __syncthreads();
+  for (uint64_t oi = d.op_begin; oi < d.op_end; ++oi) {
+    const qsim_hip_traj_op o = ops[oi];
+    if (o.kind == QSIM_HIP_TRAJ_GATE) {
+      TrajGate<NT>(s, n, o, mats, lm);
+    } else if (o.kind == QSIM_HIP_TRAJ_TILE_PASS) {
+      if (passes[o.moff].g.tb == TB) {
+        TrajTileRun<TB, true>(s, passes + o.moff, recs, lds_raw);
+      } else {
+        TrajTileRun<TB, false>(s, passes + o.moff, recs, lds_raw);
+      }
+      oi += o.k;  // the pass's gates
+    } else {
+      TrajDecide<NT>(s, n, o, mats, dec_out, red);
     }
   }
+  TrajFinish<NT>(s, d, pairs, zz_out, norm_out, red);
 }
 
 struct TrajBatch {
@@ -215,7 +498,12 @@ struct TrajBatch {
   uint32_t* pairs = nullptr;
   float2* pool = nullptr;
   double* zz = nullptr;
+  double* norms = nullptr;  // per trajectory (device order)
+  std::vector<uint8_t> normalize;
   int32_t* dec = nullptr;
+  TrajTilePass* passes = nullptr;
+  uint16_t* recs = nullptr;
+  unsigned max_tile_bits = 0;  // over the batch's tile passes
   std::vector<uint64_t> state_offset;
   std::vector<unsigned> nq;
   hipStream_t gstream = nullptr;
@@ -227,7 +515,40 @@ struct TrajBatch {
 template <int NT>
 void LaunchTraj(TrajBatch& b, hipStream_t st) {
   hipLaunchKernelGGL(TrajectoryKernel<NT>, dim3(b.num_traj), dim3(NT), 0, st,
-                     b.descs, b.ops, b.mats, b.pairs, b.pool, b.zz, b.dec);
+                     b.descs, b.ops, b.mats, b.pairs, b.pool, b.zz, b.norms,
+                     b.dec);
+}
+
+// The tiled kernel needs more than 64 KiB of dynamic LDS (allowed once per
+// process, outside any stream capture) and addresses its tile from LDS
+// offset 0, which holds only with no static LDS ahead of the dynamic block.
+int TileKernelReady() {
+  static std::once_flag once;
+  static hipError_t attr = hipSuccess;
+  static size_t static_lds = 0;
+  std::call_once(once, [] {
+    const struct {
+      const void* fn;
+      size_t lds;
+    } fns[2] = {
+        {reinterpret_cast<const void*>(&TrajTileKernel<13>), BlockLds<13>()},
+        {reinterpret_cast<const void*>(&TrajTileKernel<14>), BlockLds<14>()}};
+    for (const auto& f : fns) {
+      hipFuncAttributes fa{};
+      if (attr == hipSuccess) attr = hipFuncGetAttributes(&fa, f.fn);
+      static_lds = std::max(static_lds, fa.sharedSizeBytes);
+      if (attr == hipSuccess) {
+        attr = hipFuncSetAttribute(
+            f.fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(f.lds));
+      }
+    }
+  });
+  QHIP_CHECK(attr);
+  if (static_lds != 0) {
+    SetError("tiled trajectory kernel has %zu bytes of static LDS", static_lds);
+    return QSIM_HIP_ERR_RUNTIME;
+  }
+  return QSIM_HIP_OK;
 }
 
 int Launch(TrajBatch& b, unsigned threads, hipStream_t st) {
@@ -235,12 +556,84 @@ int Launch(TrajBatch& b, unsigned threads, hipStream_t st) {
     case 256: LaunchTraj<256>(b, st); break;
     case 512: LaunchTraj<512>(b, st); break;
     case 1024: LaunchTraj<1024>(b, st); break;
+    case QSIM_HIP_TRAJ_TILED:
+      QHIP_TRY(TileKernelReady());
+      // The 14-qubit shape when a pass needs it.
+      if (b.max_tile_bits == 14) {
+        hipLaunchKernelGGL(TrajTileKernel<14>, dim3(b.num_traj),
+                           dim3(BlockShape<14>::NT), BlockLds<14>(), st,
+                           b.descs, b.ops, b.mats, b.passes, b.recs, b.pairs,
+                           b.pool, b.zz, b.norms, b.dec);
+      } else {
+        hipLaunchKernelGGL(TrajTileKernel<13>, dim3(b.num_traj),
+                           dim3(BlockShape<13>::NT), BlockLds<13>(), st,
+                           b.descs, b.ops, b.mats, b.passes, b.recs, b.pairs,
+                           b.pool, b.zz, b.norms, b.dec);
+      }
+      QHIP_LAUNCHED("TrajTileKernel");
+      return QSIM_HIP_OK;
     default:
-      SetError("trajectory batches run with 256, 512 or 1024 threads, not %u",
-               threads);
+      SetError("trajectory batches run with 256, 512 or 1024 threads or "
+               "tiled (%u), not %u", unsigned(QSIM_HIP_TRAJ_TILED), threads);
       return QSIM_HIP_ERR_ARG;
   }
   QHIP_LAUNCHED("TrajectoryKernel");
+  return QSIM_HIP_OK;
+}
+
+// The device form of the tile pass whose header is o[0] (its gates o[1..k],
+// already validated): geometry, program and op records as the circuit
+// kernel's blocked passes build them, from the gates' FP32 matrices (exact
+// in FP64, and exact again in the three bf16 parts).  Passes with the same
+// tile, gates and matrices -- most passes of the trajectories of one
+// circuit -- share one device copy.
+int BuildTilePass(unsigned n, const qsim_hip_traj_op* o, const float* mats,
+                  std::vector<TrajTilePass>& passes,
+                  std::vector<uint16_t>& recs,
+                  std::unordered_map<std::string, uint32_t>& seen,
+                  uint32_t* index) {
+  const unsigned nops = o[0].k;
+  std::string key;
+  auto put = [&](const void* p, size_t bytes) {
+    key.append(static_cast<const char*>(p), bytes);
+  };
+  put(&n, sizeof(n));
+  put(o[0].q, 2 * sizeof(uint32_t));
+  for (unsigned j = 1; j <= nops; ++j) {
+    put(&o[j].k, sizeof(uint32_t));
+    put(o[j].q, o[j].k * sizeof(uint32_t));
+    put(mats + o[j].moff, (size_t{2} << (2 * o[j].k)) * sizeof(float));
+  }
+  auto it = seen.find(key);
+  if (it != seen.end()) {
+    *index = it->second;
+    return QSIM_HIP_OK;
+  }
+  const uint64_t mask = uint64_t(o[0].q[0]) | (uint64_t(o[0].q[1]) << 32);
+  unsigned tile[kBlockMaxBits], nt = 0;
+  for (unsigned q = 0; q < n; ++q) {
+    if ((mask >> q) & 1u) tile[nt++] = q;
+  }
+  std::vector<unsigned> ks(nops), qs(4 * size_t(nops), 0);
+  std::vector<double> m64;
+  std::vector<size_t> moff(nops);
+  for (unsigned j = 0; j < nops; ++j) {
+    const qsim_hip_traj_op& gop = o[j + 1];
+    ks[j] = gop.k;
+    for (unsigned b = 0; b < gop.k; ++b) qs[4 * j + b] = gop.q[b];
+    moff[j] = m64.size();
+    const size_t nf = size_t{2} << (2 * gop.k);
+    m64.insert(m64.end(), mats + gop.moff, mats + gop.moff + nf);
+  }
+  TrajTilePass tp{};
+  std::vector<uint16_t> fb;
+  QHIP_TRY(PrepareGateBlock(n, tile, nt, nops, ks.data(), qs.data(),
+                            m64.data(), moff.data(), tp.g, tp.p, fb));
+  tp.rec = recs.size() / (2 * size_t(kOpWords));
+  recs.insert(recs.end(), fb.begin(), fb.end());
+  *index = uint32_t(passes.size());
+  seen.emplace(std::move(key), *index);
+  passes.push_back(tp);
   return QSIM_HIP_OK;
 }
 
@@ -269,7 +662,10 @@ void Release(TrajBatch* b) {
   (void)hipFree(b->pairs);
   (void)hipFree(b->pool);
   (void)hipFree(b->zz);
+  (void)hipFree(b->norms);
   (void)hipFree(b->dec);
+  (void)hipFree(b->passes);
+  (void)hipFree(b->recs);
   delete b;
 }
 
@@ -319,6 +715,16 @@ int qsim_hip_traj_prepare(const qsim_hip_traj_desc* trajs, unsigned num_traj,
       } else if (o.kind == QSIM_HIP_TRAJ_DAMP_DECIDE) {
         ok = o.q[0] < n && uint64_t(o.moff) + 8 <= num_floats &&
              uint64_t(o.moff2) + 8 <= num_floats && o.slot < num_decisions;
+      } else if (o.kind == QSIM_HIP_TRAJ_TILE_PASS) {
+        // The header of a pass over the gates that follow it (each checked
+        // as a gate in its turn; their tile membership when the pass is
+        // built).
+        const uint64_t mask = uint64_t(o.q[0]) | (uint64_t(o.q[1]) << 32);
+        const unsigned nt = unsigned(__builtin_popcountll(mask));
+        ok = o.k >= 1 && o.k <= kBlockMaxOps && i + o.k < d[t].op_end &&
+             (mask >> n) == 0 && nt >= kBlockMinBits && nt <= kBlockMaxBits;
+        for (unsigned j = 1; ok && j <= o.k; ++j)
+          ok = ops[i + j].kind == QSIM_HIP_TRAJ_GATE;
       } else {
         ok = false;
       }
@@ -336,21 +742,45 @@ int qsim_hip_traj_prepare(const qsim_hip_traj_desc* trajs, unsigned num_traj,
       }
     }
   }
+  // Tile passes to their device form; the header's moff becomes the pass
+  // index.
+  std::vector<qsim_hip_traj_op> ov(ops, ops + num_ops);
+  std::vector<TrajTilePass> passes;
+  std::vector<uint16_t> recs;
+  {
+    std::unordered_map<std::string, uint32_t> seen;
+    for (unsigned t = 0; t < num_traj; ++t) {
+      for (uint64_t i = d[t].op_begin; i < d[t].op_end; ++i) {
+        if (ov[i].kind != QSIM_HIP_TRAJ_TILE_PASS) continue;
+        const int rc = BuildTilePass(d[t].num_qubits, ops + i, matrices,
+                                     passes, recs, seen, &ov[i].moff);
+        if (rc != QSIM_HIP_OK) return rc;
+      }
+    }
+    if (!passes.empty()) QHIP_TRY(TileKernelReady());
+  }
   auto* b = new TrajBatch;
   b->num_traj = num_traj;
+  for (const TrajTilePass& tp : passes) {
+    b->max_tile_bits = std::max(b->max_tile_bits, tp.g.tb);
+  }
   b->num_pairs = num_pairs;
   b->num_decisions = num_decisions;
   for (unsigned t = 0; t < num_traj; ++t) {
     b->state_offset.push_back(d[t].state_offset);
     b->nq.push_back(d[t].num_qubits);
+    b->normalize.push_back(d[t].pad & 1u);
   }
   int rc = Upload(&b->descs, d.data(), d.size());
-  if (rc == QSIM_HIP_OK) rc = Upload(&b->ops, ops, num_ops);
+  if (rc == QSIM_HIP_OK) rc = Upload(&b->ops, ov.data(), num_ops);
+  if (rc == QSIM_HIP_OK) rc = Upload(&b->passes, passes.data(), passes.size());
+  if (rc == QSIM_HIP_OK) rc = Upload(&b->recs, recs.data(), recs.size());
   if (rc == QSIM_HIP_OK) rc = Upload(&b->mats, matrices, num_floats);
   if (rc == QSIM_HIP_OK) rc = Upload(&b->pairs, pairs, size_t(2) * num_pairs);
   if (rc == QSIM_HIP_OK) {
     hipError_t e = hipMalloc(&b->pool, pool * sizeof(float2));
     if (e == hipSuccess) e = hipMalloc(&b->zz, std::max<size_t>(num_pairs, 1) * 8);
+    if (e == hipSuccess) e = hipMalloc(&b->norms, size_t(num_traj) * 8);
     if (e == hipSuccess)
       e = hipMalloc(&b->dec, std::max<size_t>(num_decisions, 1) * 4);
     if (e != hipSuccess) rc = HipFail(e, "trajectory batch buffers");
@@ -421,9 +851,17 @@ int qsim_hip_traj_state(void* handle, unsigned t, float* out) {
     SetError("trajectory batch: bad trajectory index");
     return QSIM_HIP_ERR_ARG;
   }
+  const size_t count = size_t{2} << b->nq[t];
   QHIP_CHECK(hipMemcpy(out, b->pool + b->state_offset[t],
-                       (size_t{1} << b->nq[t]) * sizeof(float2),
-                       hipMemcpyDeviceToHost));
+                       count * sizeof(float), hipMemcpyDeviceToHost));
+  if (b->normalize[t]) {
+    // The end-of-trajectory normalisation (the norm from the final sweep).
+    double nrm = 0.0;
+    QHIP_CHECK(hipMemcpy(&nrm, b->norms + t, sizeof(double),
+                         hipMemcpyDeviceToHost));
+    const float a = float(1.0 / std::sqrt(nrm));
+    for (size_t i = 0; i < count; ++i) out[i] *= a;
+  }
   return QSIM_HIP_OK;
 }
 

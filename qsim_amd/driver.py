@@ -62,6 +62,7 @@ def driver_lib():
                                           ctypes.c_double]),
                 ("qsim_traj_num_channels", u, [vp, u]),
                 ("qsim_traj_plan", i, [vp, up, vp, u, u]),
+                ("qsim_traj_plan_tiled", i, [vp, up, vp, u, u, u]),
                 ("qsim_traj_run", i, [vp, u, i]),
                 ("qsim_traj_zz", i, [vp, u, vp, u, up]),
                 ("qsim_traj_kraus", i, [vp, u, vp, u, up]),

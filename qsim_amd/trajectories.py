@@ -19,6 +19,16 @@ from . import _check, circuits
 from .driver import driver_lib
 
 
+# qsim_hip_traj_run's kernel choice for tile passes (QSIM_HIP_TRAJ_TILED).
+TILED = 1
+
+# Autotuner candidates: (threads, plan); plan is a fusion size or a tile
+# spec ("tile", tile_bits, low_bits).
+DEFAULT_CANDIDATES = ((TILED, ("tile", 14, 5)), (TILED, ("tile", 13, 5)),
+                      (TILED, ("tile", 12, 5)), (TILED, ("tile", 13, 6)),
+                      (256, 4), (512, 4), (1024, 4), (1024, 3))
+
+
 class TrajectoryBatch:
     """A set of noisy grid circuits and a planned batch of trajectories."""
 
@@ -51,17 +61,31 @@ class TrajectoryBatch:
         return driver_lib().qsim_traj_num_channels(self.h, circuit)
 
     def plan(self, circuit_of, reps, max_fused_size=4) -> float:
-        """Host planning (sampling + fusion); returns its wall seconds."""
+        """Host planning (sampling + fusion); returns its wall seconds.
+
+        ``max_fused_size`` is 1..4 (fused gates, one sweep of the state
+        each), or a tile spec ``("tile", tile_bits, low_bits)``: runs of
+        gates become tile passes (one sweep per pass) for the tiled kernel
+        (``run(TILED)``)."""
         co = np.ascontiguousarray(circuit_of, np.uint32)
         rp = np.ascontiguousarray(reps, np.uint64)
+        cp = co.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
         t0 = time.time()
-        _check(driver_lib().qsim_traj_plan(
-            self.h, co.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
-            rp.ctypes.data, len(co), max_fused_size))
+        if isinstance(max_fused_size, tuple):
+            kind, tile_bits, low_bits = max_fused_size
+            if kind != "tile":
+                raise ValueError(f"unknown plan {max_fused_size!r}")
+            _check(driver_lib().qsim_traj_plan_tiled(
+                self.h, cp, rp.ctypes.data, len(co), tile_bits, low_bits))
+        else:
+            _check(driver_lib().qsim_traj_plan(
+                self.h, cp, rp.ctypes.data, len(co), max_fused_size))
         self.circuit_of, self.reps = co, rp
         return time.time() - t0
 
     def run(self, threads=512, use_graph=False):
+        """One launch of the planned batch: ``threads`` 256, 512 or 1024
+        (per-gate kernels) or ``TILED`` (tile passes, 256 threads)."""
         _check(driver_lib().qsim_traj_run(self.h, threads, int(use_graph)))
 
     def zz(self, t: int) -> np.ndarray:
@@ -85,11 +109,13 @@ class TrajectoryBatch:
         _check(driver_lib().qsim_traj_state(self.h, t, out.ctypes.data))
         return out
 
-    def autotune(self, candidates=((256, 4), (512, 4), (1024, 4), (512, 3),
-                                   (512, 2)), sample=None, log=None):
-        """Times each (threads, max_fused_size) on a sample of the planned
-        trajectories and returns the fastest; the batch is left planned
-        with that fusion size."""
+    def autotune(self, candidates=DEFAULT_CANDIDATES, sample=None, log=None):
+        """Times each (threads, plan) -- a per-gate kernel shape with a
+        fusion size, or the tiled kernel with a tile shape -- on a sample
+        of the planned trajectories (the whole batch by default: one
+        workgroup runs one trajectory, so a small sample measures latency,
+        not throughput) and returns the fastest; the batch is left planned
+        that way."""
         co, rp = self.circuit_of, self.reps
         if sample is not None and sample < len(co):
             idx = np.linspace(0, len(co) - 1, sample).astype(int)
@@ -98,18 +124,20 @@ class TrajectoryBatch:
             sco, srp = co, rp
         best = None
         results = []
-        for threads, kmax in candidates:
-            self.plan(sco, srp, kmax)
+        for threads, plan in candidates:
+            self.plan(sco, srp, plan)
             self.run(threads)
             t0 = time.time()
             self.run(threads)
             dt = time.time() - t0
-            results.append({"threads": threads, "max_fused_size": kmax,
+            kernel = "tiled" if threads == TILED else f"{threads} threads"
+            results.append({"kernel": kernel, "plan": describe_plan(plan),
                             "s": dt})
             if log:
-                log(f"autotune threads={threads} k={kmax}: {dt * 1e3:.1f} ms")
+                log(f"autotune {kernel}, {describe_plan(plan)}: "
+                    f"{dt * 1e3:.1f} ms")
             if best is None or dt < best[2]:
-                best = (threads, kmax, dt)
+                best = (threads, plan, dt)
         self.plan(co, rp, best[1])
         return best[0], best[1], results
 
@@ -123,6 +151,12 @@ class TrajectoryBatch:
             self.free()
         except Exception:
             pass
+
+
+def describe_plan(plan) -> str:
+    if isinstance(plan, tuple):
+        return f"tile passes {plan[1]}/{plan[2]}"
+    return f"fused k={plan}"
 
 
 def grid_pairs(num_qubits):

@@ -6,15 +6,20 @@ Bars: the Kraus index of every channel identical to the CPU run (the host
 replays the same mt19937 draws; the state-dependent damping decisions are
 taken on the device); <Z_a Z_b> on grid neighbours within 2e-5 of the CPU
 values (FP32 states on both sides, FP64 sums); graph replay bit-identical to
-the plain launch.
+the plain launch.  The tiled kernel (tile passes: runs of gates applied to
+LDS-staged tiles in split-BF16 MFMA arithmetic, one stream of the state per
+pass) is held to the same bars.
 """
 
+import ctypes
 import os
 
 import numpy as np
 import pytest
 
 import oracle
+import qsim_amd
+from helpers import random_matrix, to_interleaved
 from qsim_amd import circuits, trajectories
 
 pytestmark = [pytest.mark.gpu,
@@ -103,3 +108,141 @@ def test_final_state_is_normalised():
     for t in range(2):
         s = b.state(t).astype(np.float64)
         assert abs(np.sum(s * s) - 1.0) < 1e-5
+
+
+TILED = trajectories.TILED
+
+
+# 14-qubit tiles select the 512-thread shape; there the 12- and 13-qubit
+# states run partial tiles.
+@pytest.mark.parametrize("plan", [("tile", 9, 5), ("tile", 13, 5),
+                                  ("tile", 11, 3), ("tile", 14, 5),
+                                  ("tile", 14, 9)])
+def test_tiled_kernel_matches_qtrajectory(tmp_path, plan):
+    check_batch(tmp_path, [12, 14, 16], 8, 8, p=0.05, gamma=0.1,
+                threads=TILED, kmax=plan)
+
+
+@pytest.mark.parametrize("tile_bits", [13, 14])
+def test_tiled_mixed_18_20_22_batch(tmp_path, tile_bits):
+    check_batch(tmp_path, [18, 20, 22], 12, 4, p=1e-3, gamma=5e-3,
+                threads=TILED, kmax=("tile", tile_bits, 5))
+
+
+def test_per_gate_kernels_run_tiled_plans():
+    # The per-gate kernels skip the pass headers and apply the gates one by
+    # one; a 6-qubit trajectory keeps fused gates in a tiled plan.  Same
+    # Kraus choices and <ZZ> as the tiled kernel; final states within the
+    # blocked passes' bar (1e-4 relative L2: split-BF16 against FP32
+    # arithmetic over some 30 gates).
+    b = trajectories.TrajectoryBatch([6, 12, 16], 6, p_depol=0.05,
+                                     gamma_damp=0.1)
+    co, reps = [0, 1, 2, 0, 1, 2], [11, 12, 13, 14, 15, 16]
+    b.plan(co, reps, ("tile", 12, 5))
+    b.run(TILED)
+    tiled = [(b.zz(t), b.kraus(t), b.state(t)) for t in range(6)]
+    b.run(512)
+    for t in range(6):
+        zz, k, st = tiled[t]
+        assert np.array_equal(b.kraus(t), k)
+        assert np.max(np.abs(b.zz(t) - zz)) < 2e-5
+        ref = b.state(t)
+        assert np.linalg.norm(st - ref) < 1e-4 * np.linalg.norm(ref)
+
+
+def test_tiled_graph_replay_is_identical():
+    b = trajectories.TrajectoryBatch([12, 14], 6, p_depol=0.05, gamma_damp=0.1)
+    b.plan([0, 1, 0, 1], [5, 6, 7, 8], ("tile", 10, 4))
+    b.run(TILED)
+    ref = [b.zz(t) for t in range(4)]
+    kref = [b.kraus(t) for t in range(4)]
+    for _ in range(3):
+        b.run(TILED, use_graph=True)
+        for t in range(4):
+            assert np.array_equal(b.zz(t), ref[t])
+            assert np.array_equal(b.kraus(t), kref[t])
+
+
+# The C-ABI structs (include/qsim_hip.h).
+OP = np.dtype([("kind", "<u4"), ("k", "<u4"), ("q", "<u4", (4,)),
+               ("moff", "<u4"), ("moff2", "<u4"), ("r", "<f4"),
+               ("kdk1", "<f4"), ("slot", "<u4"), ("pad", "<u4")])
+DESC = np.dtype([("num_qubits", "<u4"), ("num_pairs", "<u4"),
+                 ("state_offset", "<u8"), ("op_begin", "<u8"),
+                 ("op_end", "<u8"), ("pair_offset", "<u4"), ("pad", "<u4")])
+
+
+def raw_batch(n, ops, mats):
+    d = np.zeros(1, DESC)
+    d[0]["num_qubits"] = n
+    d[0]["op_end"] = len(ops)
+    h = ctypes.c_void_p()
+    pairs = np.zeros(2, np.uint32)
+    rc = qsim_amd.lib().qsim_hip_traj_prepare(
+        d.ctypes.data, 1, ops.ctypes.data, len(ops), mats.ctypes.data,
+        len(mats), pairs.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), 0,
+        0, ctypes.byref(h))
+    return rc, h
+
+
+def tile_stream(mask, gates):
+    """A tile-pass header (QSIM_HIP_TRAJ_TILE_PASS) and its gates."""
+    ops = np.zeros(1 + len(gates), OP)
+    ops["kind"][0] = 2
+    ops["k"][0] = len(gates)
+    ops["q"][0, :2] = [mask & 0xFFFFFFFF, mask >> 32]
+    mats = []
+    rng = np.random.default_rng(len(gates))
+    for i, qs in enumerate(gates):
+        ops["kind"][1 + i] = 0
+        ops["k"][1 + i] = len(qs)
+        ops["q"][1 + i, :len(qs)] = qs
+        ops["moff"][1 + i] = sum(len(m) for m in mats)
+        mats.append(to_interleaved(random_matrix(len(qs), rng), np.float32))
+    return ops, np.concatenate(mats).astype(np.float32)
+
+
+def test_tile_pass_header_runs_raw():
+    # A hand-built stream: the tiled kernel and the per-gate kernel give
+    # the same state.
+    n = 12
+    ops, mats = tile_stream(0b1111111111, [[3], [4, 9], [0, 1, 2, 5]])
+    rc, h = raw_batch(n, ops, mats)
+    assert rc == 0
+    L = qsim_amd.lib()
+    out = []
+    for threads in (TILED, 256):
+        assert L.qsim_hip_traj_run(h, threads, 0) == 0
+        st = np.zeros(2 << n, np.float32)
+        assert L.qsim_hip_traj_state(h, 0, st.ctypes.data) == 0
+        out.append(st)
+    L.qsim_hip_traj_free(h)
+    assert np.linalg.norm(out[0] - out[1]) < 1e-5 * np.linalg.norm(out[1])
+
+
+@pytest.mark.parametrize("bad", ["k0", "past_end", "mask_above_n", "small",
+                                 "big", "no_low_run", "outside", "decision"])
+def test_malformed_tile_passes_are_rejected(bad):
+    n = 16
+    mask = (1 << 10) - 1
+    gates = [[3], [4, 9]]
+    ops, mats = tile_stream(mask, gates)
+    if bad == "k0":
+        ops["k"][0] = 0
+    elif bad == "past_end":
+        ops["k"][0] = 3
+    elif bad == "mask_above_n":
+        ops["q"][0, 0] = mask | (1 << 16)
+    elif bad == "small":
+        ops["q"][0, 0] = 0b1000011111  # 6 tile qubits
+    elif bad == "big":
+        ops["q"][0, 0] = (1 << 15) - 1  # 15 tile qubits
+    elif bad == "no_low_run":
+        ops["q"][0, 0] = mask & ~0b100 | (1 << 12)
+    elif bad == "outside":
+        ops["q"][2, 1] = 11
+    elif bad == "decision":
+        ops["kind"][2] = 1
+    rc, h = raw_batch(n, ops, mats)
+    assert rc == qsim_amd.ERR_ARG, bad
+    assert not h.value
