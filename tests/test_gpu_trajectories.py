@@ -150,9 +150,12 @@ def test_per_gate_kernels_run_tiled_plans():
         assert np.linalg.norm(st - ref) < 1e-4 * np.linalg.norm(ref)
 
 
-def test_tiled_graph_replay_is_identical():
+@pytest.mark.parametrize("tile_bits", [10, 14])
+def test_tiled_graph_replay_is_identical(tile_bits):
+    # 10-qubit tiles run the 256-thread shape, 14-qubit tiles the 512-thread
+    # one (the 12-qubit states as partial tiles).
     b = trajectories.TrajectoryBatch([12, 14], 6, p_depol=0.05, gamma_damp=0.1)
-    b.plan([0, 1, 0, 1], [5, 6, 7, 8], ("tile", 10, 4))
+    b.plan([0, 1, 0, 1], [5, 6, 7, 8], ("tile", tile_bits, 4))
     b.run(TILED)
     ref = [b.zz(t) for t in range(4)]
     kref = [b.kraus(t) for t in range(4)]
@@ -161,6 +164,23 @@ def test_tiled_graph_replay_is_identical():
         for t in range(4):
             assert np.array_equal(b.zz(t), ref[t])
             assert np.array_equal(b.kraus(t), kref[t])
+
+
+def test_replanning_reuses_the_batch():
+    # A batch planned and run one way, then re-planned the other way (its
+    # device buffers replaced), gives the same Kraus choices and <ZZ>.
+    b = trajectories.TrajectoryBatch([12, 16], 6, p_depol=0.05, gamma_damp=0.1)
+    co, reps = [0, 1, 1, 0], [21, 22, 23, 24]
+    out = []
+    for threads, plan in ((TILED, ("tile", 14, 5)), (512, 4), (TILED, ("tile", 12, 5))):
+        b.plan(co, reps, plan)
+        b.run(threads, use_graph=True)
+        out.append([(b.kraus(t), b.zz(t)) for t in range(4)])
+    for other in out[1:]:
+        for (k0, z0), (k1, z1) in zip(out[0], other):
+            assert np.array_equal(k0, k1)
+            assert np.max(np.abs(z0 - z1)) < 2e-5
+    b.free()
 
 
 # The C-ABI structs (include/qsim_hip.h).
