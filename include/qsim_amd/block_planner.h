@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <functional>
 #include <set>
 #include <variant>
 #include <vector>
@@ -109,6 +110,36 @@ class Frontier {
     return cnt;
   }
 
+  // The closure continued from `ptr` (already closed over `in` minus
+  // `seeds`) after the qubits `seeds` joined `in`: only frontier gates of
+  // qubits whose frontier moved (or that joined) can become applicable, so a
+  // worklist of those qubits reaches the same fixpoint as Closure.
+  size_t ClosureFrom(const std::vector<char>& in, std::vector<size_t>& ptr,
+                     const std::vector<unsigned>& seeds) const {
+    std::vector<unsigned> work(seeds);
+    size_t cnt = 0;
+    while (!work.empty()) {
+      const unsigned q = work.back();
+      work.pop_back();
+      if (!in[q]) continue;
+      while (ptr[q] < per_q_[q].size()) {
+        const size_t i = per_q_[q][ptr[q]];
+        bool ok = true;
+        for (unsigned x : gq_[i]) {
+          ok = ok && in[x] && ptr[x] < per_q_[x].size() &&
+               per_q_[x][ptr[x]] == i;
+        }
+        if (!ok) break;
+        for (unsigned x : gq_[i]) {
+          ++ptr[x];
+          if (x != q) work.push_back(x);
+        }
+        ++cnt;
+      }
+    }
+    return cnt;
+  }
+
   // Chooses a tile set (low qubits + greedy additions) and takes the gates it
   // unlocks.  Returns them in circuit order (ascending index).
   std::vector<size_t> NextPass(unsigned tile_bits, unsigned low_bits,
@@ -120,9 +151,10 @@ class Frontier {
       in[q] = 1;
       ++size;
     }
+    // p0: the frontier after every gate the current set closes over.
+    std::vector<size_t> p0 = ptr_;
+    Closure(in, p0, nullptr);
     while (size < tile_bits) {
-      std::vector<size_t> p0 = ptr_;
-      const size_t base = Closure(in, p0, nullptr);
       // Candidates: single qubits, and the missing qubits of frontier gates.
       std::vector<std::vector<unsigned>> cands;
       for (unsigned q = 0; q < n; ++q)
@@ -140,9 +172,10 @@ class Frontier {
       for (const auto& c : cands) {
         if (size + c.size() > tile_bits) continue;
         for (unsigned x : c) in[x] = 1;
-        std::vector<size_t> p1 = ptr_;
-        const double gain =
-            double(Closure(in, p1, nullptr) - base) / double(c.size());
+        // The closure is a unique fixpoint and grows with the qubit set, so
+        // the candidate's extra gates are the closure continued from p0.
+        std::vector<size_t> p1 = p0;
+        const double gain = double(ClosureFrom(in, p1, c)) / double(c.size());
         for (unsigned x : c) in[x] = 0;
         if (gain > best) {
           best = gain;
@@ -152,6 +185,7 @@ class Frontier {
       if (pick == nullptr) break;
       for (unsigned x : *pick) in[x] = 1;
       size += pick->size();
+      ClosureFrom(in, p0, *pick);
     }
     std::vector<size_t> taken;
     Closure(in, ptr_, &taken);
@@ -260,9 +294,13 @@ inline void MergeTileGates(std::vector<TileGate>& g) {
 
 // Plans blocked passes for ops of a circuit on `num_qubits` qubits.  Returns
 // false if the circuit is too small for blocked passes (num_qubits < 9).
+// `fill`, if given, supplies each fused gate's matrix (qsim's fuser then
+// skips its own products): a caller planning many similar gate lists can
+// reuse matrices it has already formed.
 template <typename IO, typename FP>
 bool PlanBlocks(const BlockPlannerParameter& param, unsigned num_qubits,
-                const std::vector<Operation<FP>>& ops, BlockPlan& plan) {
+                const std::vector<Operation<FP>>& ops, BlockPlan& plan,
+                const std::function<void(FusedGate<FP>&)>* fill = nullptr) {
   plan = BlockPlan{};
   const unsigned tb = std::min(param.tile_bits, num_qubits);
   if (tb < 9 || tb > 14) return false;
@@ -290,11 +328,13 @@ bool PlanBlocks(const BlockPlannerParameter& param, unsigned num_qubits,
         const std::vector<unsigned> no_splits;
         auto fused =
             MultiQubitGateFuser<IO>::template FuseGates<const Operation<FP>*>(
-                fp, num_qubits, ptrs.cbegin(), ptrs.cend(), no_splits, true);
+                fp, num_qubits, ptrs.cbegin(), ptrs.cend(), no_splits,
+                fill == nullptr);
         std::vector<detail::TileGate> tg;
-        for (const auto& f : fused) {
-          const auto* fg = std::get_if<FusedGate<FP>>(&f);
+        for (auto& f : fused) {
+          auto* fg = std::get_if<FusedGate<FP>>(&f);
           if (fg == nullptr) return false;  // only gates were passed in
+          if (fill != nullptr) (*fill)(*fg);
           tg.push_back({fg->qubits,
                         std::vector<double>(fg->matrix.begin(),
                                             fg->matrix.end())});

@@ -20,10 +20,13 @@
 // every cycle (gate times 2c and 2c + 1), an amplitude-damping channel
 // (AmplitudeDampingChannel, lib/channels_cirq.h:274-300) on every qubit.
 
+#include <omp.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <random>
 #include <sstream>
@@ -65,6 +68,7 @@ struct TrajPlan {
   uint64_t rep = 0;
   std::vector<qsim_hip_traj_op> ops;
   std::vector<std::vector<float>> mats;  // per gate op, FP32 matrix
+  std::vector<uint64_t> mhash;           // per op, MatrixHash of mats[i]
   std::vector<int32_t> kraus;            // per channel: index, or -2 - slot
   unsigned num_decisions = 0;
   bool normalize_last = false;           // non-unitary at the end
@@ -152,14 +156,58 @@ struct FusionSpec {
   unsigned low_bits = 5;
 };
 
+// Fused matrices by fused gate: its qubits and its component gates' qubits
+// and matrices.  The trajectories of one circuit share most of their fused
+// gates (they differ where a Kraus operator other than the likeliest was
+// drawn), so qsim's product (CalculateFusedMatrix, lib/fuser.h:150-179) runs
+// once per distinct gate and the others are copies of its result.  One
+// cache per planning thread.
+class FusedMatrixCache {
+ public:
+  void Fill(FusedGate<double>& fg) {
+    key_.clear();
+    Put(fg.qubits.data(), fg.qubits.size() * sizeof(unsigned));
+    for (const auto& pgate : fg.gates) {
+      const auto* pg = OpGetAlternative<Gate<double>>(pgate);
+      const auto& pq = OpQubits(pgate);
+      const auto& pm =
+          pg ? pg->matrix : OpGetAlternative<DecomposedGate<double>>(pgate)->matrix;
+      const size_t nq = pq.size(), nm = pm.size();
+      Put(&nq, sizeof(nq));
+      Put(pq.data(), nq * sizeof(unsigned));
+      Put(&nm, sizeof(nm));
+      Put(pm.data(), nm * sizeof(double));
+    }
+    auto it = memo_.find(key_);
+    if (it != memo_.end()) {
+      fg.matrix = it->second;
+      return;
+    }
+    CalculateFusedMatrix(fg);
+    if (memo_.size() >= kMaxEntries) memo_.clear();
+    memo_.emplace(key_, fg.matrix);
+  }
+
+ private:
+  static constexpr size_t kMaxEntries = 1 << 16;
+  void Put(const void* p, size_t bytes) {
+    key_.append(static_cast<const char*>(p), bytes);
+  }
+  std::string key_;
+  std::unordered_map<std::string, Matrix<double>> memo_;
+};
+
 // Tile passes for a run of gates; false if the state is too small for them.
 bool FlushTiled(const std::vector<Operation<double>>& deferred, unsigned n,
-                const FusionSpec& spec, TrajPlan& plan) {
+                const FusionSpec& spec, FusedMatrixCache& cache,
+                TrajPlan& plan) {
   hip::BlockPlannerParameter bp;
   bp.tile_bits = spec.tile_bits;
   bp.low_bits = spec.low_bits;
   hip::BlockPlan blocks;
-  if (!hip::PlanBlocks<IO>(bp, n, deferred, blocks)) return false;
+  const std::function<void(FusedGate<double>&)> fill =
+      [&cache](FusedGate<double>& fg) { cache.Fill(fg); };
+  if (!hip::PlanBlocks<IO>(bp, n, deferred, blocks, &fill)) return false;
   for (const auto& step : blocks.steps) {
     if (!step.blocked) return false;  // only gates are deferred
   }
@@ -189,11 +237,11 @@ bool FlushTiled(const std::vector<Operation<double>>& deferred, unsigned n,
 }
 
 void Flush(std::vector<Operation<double>>& deferred, unsigned n,
-           const FusionSpec& spec, TrajPlan& plan) {
+           const FusionSpec& spec, FusedMatrixCache& cache, TrajPlan& plan) {
   if (deferred.empty()) return;
   if (spec.tile_bits >= 9) {
     const size_t nops = plan.ops.size();
-    if (FlushTiled(deferred, n, spec, plan)) {
+    if (FlushTiled(deferred, n, spec, cache, plan)) {
       deferred.clear();
       return;
     }
@@ -204,10 +252,11 @@ void Flush(std::vector<Operation<double>>& deferred, unsigned n,
   MultiQubitGateFuser<IO>::Parameter fp;
   fp.max_fused_size = max_fused;
   fp.verbosity = 0;
-  auto fused = MultiQubitGateFuser<IO>::FuseGates(fp, n, deferred);
-  for (const auto& f : fused) {
-    const auto* fg = std::get_if<FusedGate<double>>(&f);
+  auto fused = MultiQubitGateFuser<IO>::FuseGates(fp, n, deferred, false);
+  for (auto& f : fused) {
+    auto* fg = std::get_if<FusedGate<double>>(&f);
     if (fg == nullptr) continue;
+    cache.Fill(*fg);
     qsim_hip_traj_op o{};
     o.kind = QSIM_HIP_TRAJ_GATE;
     o.k = fg->qubits.size();
@@ -218,9 +267,23 @@ void Flush(std::vector<Operation<double>>& deferred, unsigned n,
   deferred.clear();
 }
 
+// 64-bit hash of an FP32 matrix's bits (for sharing identical matrices
+// across the batch; equal hashes are confirmed by comparing the bits).
+uint64_t MatrixHash(const std::vector<float>& m) {
+  uint64_t h = 0x9e3779b97f4a7c15ull ^ m.size();
+  for (float f : m) {
+    uint32_t u;
+    std::memcpy(&u, &f, sizeof(u));
+    h = (h ^ u) * 0xff51afd7ed558ccdull;
+    h ^= h >> 33;
+  }
+  return h;
+}
+
 // RunIteration's sampling (lib/qtrajectory.h:270-356) for one trajectory.
 void PlanTrajectory(const NoisyCircuit& nc, uint64_t rep,
-                    const FusionSpec& spec, TrajPlan& plan) {
+                    const FusionSpec& spec, FusedMatrixCache& cache,
+                    TrajPlan& plan) {
   const unsigned n = nc.num_qubits;
   std::mt19937 rgen(rep);
   std::uniform_real_distribution<double> distr(0.0, 1.0);
@@ -255,7 +318,7 @@ void PlanTrajectory(const NoisyCircuit& nc, uint64_t rep,
     }
     // The bounds do not decide: the state does (two-operator damping).  A
     // non-unitary history is normalised first, as NormalizeState does.
-    Flush(deferred, n, spec, plan);
+    Flush(deferred, n, spec, cache, plan);
     qsim_hip_traj_op o{};
     o.kind = QSIM_HIP_TRAJ_DAMP_DECIDE;
     o.q[0] = ch.qubits[0];
@@ -268,8 +331,12 @@ void PlanTrajectory(const NoisyCircuit& nc, uint64_t rep,
     plan.kraus.push_back(-2 - int32_t(o.slot));
     unitary = false;
   }
-  Flush(deferred, n, spec, plan);
+  Flush(deferred, n, spec, cache, plan);
   plan.normalize_last = !unitary;
+  plan.mhash.resize(plan.mats.size());
+  for (size_t i = 0; i < plan.mats.size(); ++i) {
+    plan.mhash[i] = MatrixHash(plan.mats[i]);
+  }
 }
 
 }  // namespace
@@ -327,11 +394,13 @@ int PlanBatch(TrajBatchHost* b, const unsigned* circuit_of,
   }
   b->have_results = false;
   b->plans.assign(num_traj, TrajPlan{});
+  std::vector<FusedMatrixCache> caches(omp_get_max_threads());
 #pragma omp parallel for schedule(dynamic, 4)
   for (int t = 0; t < int(num_traj); ++t) {
     b->plans[t].circuit = circuit_of[t];
     b->plans[t].rep = reps[t];
-    PlanTrajectory(b->circuits[circuit_of[t]], reps[t], spec, b->plans[t]);
+    PlanTrajectory(b->circuits[circuit_of[t]], reps[t], spec,
+                   caches[omp_get_thread_num()], b->plans[t]);
   }
   // Device order: largest states first.
   b->order.resize(num_traj);
@@ -340,9 +409,10 @@ int PlanBatch(TrajBatchHost* b, const unsigned* circuit_of,
     return b->circuits[b->plans[x].circuit].num_qubits >
            b->circuits[b->plans[y].circuit].num_qubits;
   });
-  // Concatenate streams; identical matrices share one copy.
+  // Concatenate streams; identical matrices share one copy (found by their
+  // hashes, computed in the parallel planning, and confirmed bit for bit).
   std::vector<float> pool(b->damp_k);
-  std::unordered_map<std::string, uint32_t> seen;
+  std::unordered_map<uint64_t, std::vector<uint32_t>> seen;  // hash -> offsets
   std::vector<qsim_hip_traj_op> ops;
   std::vector<qsim_hip_traj_desc> descs;
   std::vector<uint32_t> pairs;
@@ -370,14 +440,22 @@ int PlanBatch(TrajBatchHost* b, const unsigned* circuit_of,
       qsim_hip_traj_op o = pl.ops[i];
       if (o.kind == QSIM_HIP_TRAJ_GATE) {
         const auto& m = pl.mats[i];
-        std::string key(reinterpret_cast<const char*>(m.data()),
-                        m.size() * sizeof(float));
-        auto it = seen.find(key);
-        if (it == seen.end()) {
-          it = seen.emplace(key, uint32_t(pool.size())).first;
+        const size_t bytes = m.size() * sizeof(float);
+        auto& offs = seen[pl.mhash[i]];
+        uint32_t off = ~0u;
+        for (uint32_t c : offs) {
+          if (c + m.size() <= pool.size() &&
+              std::memcmp(pool.data() + c, m.data(), bytes) == 0) {
+            off = c;
+            break;
+          }
+        }
+        if (off == ~0u) {
+          off = uint32_t(pool.size());
+          offs.push_back(off);
           pool.insert(pool.end(), m.begin(), m.end());
         }
-        o.moff = it->second;
+        o.moff = off;
       } else if (o.kind == QSIM_HIP_TRAJ_TILE_PASS) {
         o.moff = 0;
       } else {
@@ -391,6 +469,7 @@ int PlanBatch(TrajBatchHost* b, const unsigned* circuit_of,
     descs.push_back(d);
     pl.ops.clear();
     pl.mats.clear();
+    pl.mhash.clear();
   }
   b->total_pairs = npairs;
   b->total_decisions = ndec;

@@ -30,6 +30,7 @@
 #include <cmath>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <unordered_map>
 #include <vector>
@@ -581,34 +582,46 @@ int Launch(TrajBatch& b, unsigned threads, hipStream_t st) {
   return QSIM_HIP_OK;
 }
 
-// The device form of the tile pass whose header is o[0] (its gates o[1..k],
-// already validated): geometry, program and op records as the circuit
-// kernel's blocked passes build them, from the gates' FP32 matrices (exact
-// in FP64, and exact again in the three bf16 parts).  Passes with the same
-// tile, gates and matrices -- most passes of the trajectories of one
-// circuit -- share one device copy.
-int BuildTilePass(unsigned n, const qsim_hip_traj_op* o, const float* mats,
-                  std::vector<TrajTilePass>& passes,
-                  std::vector<uint16_t>& recs,
-                  std::unordered_map<std::string, uint32_t>& seen,
-                  uint32_t* index) {
-  const unsigned nops = o[0].k;
+// A distinct tile pass to build: the header of its first occurrence (its
+// gates o[1..k] follow) and the state size.
+struct TilePassJob {
+  unsigned n;
+  const qsim_hip_traj_op* o;
+};
+
+// Index of the tile pass whose header is o[0] among the batch's distinct
+// passes, adding it if new.  Passes with the same state size, tile, gate
+// qubits and matrix offsets -- most passes of the trajectories of one
+// circuit, when the caller shares identical matrices as the batch planner
+// does -- share one device copy.
+uint32_t TilePassIndex(unsigned n, const qsim_hip_traj_op* o,
+                       std::vector<TilePassJob>& jobs,
+                       std::unordered_map<std::string, uint32_t>& seen) {
   std::string key;
   auto put = [&](const void* p, size_t bytes) {
     key.append(static_cast<const char*>(p), bytes);
   };
   put(&n, sizeof(n));
   put(o[0].q, 2 * sizeof(uint32_t));
-  for (unsigned j = 1; j <= nops; ++j) {
+  for (unsigned j = 1; j <= o[0].k; ++j) {
     put(&o[j].k, sizeof(uint32_t));
     put(o[j].q, o[j].k * sizeof(uint32_t));
-    put(mats + o[j].moff, (size_t{2} << (2 * o[j].k)) * sizeof(float));
+    put(&o[j].moff, sizeof(uint32_t));
   }
-  auto it = seen.find(key);
-  if (it != seen.end()) {
-    *index = it->second;
-    return QSIM_HIP_OK;
-  }
+  auto it = seen.emplace(std::move(key), uint32_t(jobs.size())).first;
+  if (it->second == jobs.size()) jobs.push_back({n, o});
+  return it->second;
+}
+
+// The device form of a tile pass (header and gates validated, including the
+// conditions PrepareGateBlock checks): geometry, program and op records as
+// the circuit kernel's blocked passes build them, from the gates' FP32
+// matrices (exact in FP64, and exact again in the three bf16 parts); the
+// records go to rec[0 .. k kOpWords 16-bit halves).
+int BuildTilePass(const TilePassJob& job, const float* mats, TrajTilePass& tp,
+                  uint16_t* rec) {
+  const qsim_hip_traj_op* o = job.o;
+  const unsigned n = job.n, nops = o[0].k;
   const uint64_t mask = uint64_t(o[0].q[0]) | (uint64_t(o[0].q[1]) << 32);
   unsigned tile[kBlockMaxBits], nt = 0;
   for (unsigned q = 0; q < n; ++q) {
@@ -625,15 +638,47 @@ int BuildTilePass(unsigned n, const qsim_hip_traj_op* o, const float* mats,
     const size_t nf = size_t{2} << (2 * gop.k);
     m64.insert(m64.end(), mats + gop.moff, mats + gop.moff + nf);
   }
-  TrajTilePass tp{};
   std::vector<uint16_t> fb;
-  QHIP_TRY(PrepareGateBlock(n, tile, nt, nops, ks.data(), qs.data(),
-                            m64.data(), moff.data(), tp.g, tp.p, fb));
-  tp.rec = recs.size() / (2 * size_t(kOpWords));
-  recs.insert(recs.end(), fb.begin(), fb.end());
-  *index = uint32_t(passes.size());
-  seen.emplace(std::move(key), *index);
-  passes.push_back(tp);
+  const int rc = PrepareGateBlock(n, tile, nt, nops, ks.data(), qs.data(),
+                                  m64.data(), moff.data(), tp.g, tp.p, fb);
+  if (rc == QSIM_HIP_OK) std::copy(fb.begin(), fb.end(), rec);
+  return rc;
+}
+
+// Builds the distinct passes, spread over a few host threads (each pass is
+// independent; the lane-position memo is shared under its lock).
+int BuildTilePasses(const std::vector<TilePassJob>& jobs, const float* mats,
+                    std::vector<TrajTilePass>& passes,
+                    std::vector<uint16_t>& recs) {
+  constexpr size_t kRec = 2 * size_t(kOpWords);  // 16-bit halves per record
+  passes.assign(jobs.size(), TrajTilePass{});
+  size_t total = 0;
+  for (size_t j = 0; j < jobs.size(); ++j) {
+    passes[j].rec = total;
+    total += jobs[j].o[0].k;
+  }
+  recs.assign(total * kRec, 0);
+  std::vector<int> rc(jobs.size(), QSIM_HIP_OK);
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const unsigned nw = unsigned(std::min<size_t>(
+      std::min(hw, 16u), std::max<size_t>(1, jobs.size() / 64)));
+  auto work = [&](unsigned w) {
+    for (size_t j = w; j < jobs.size(); j += nw) {
+      rc[j] = BuildTilePass(jobs[j], mats, passes[j],
+                            recs.data() + passes[j].rec * kRec);
+    }
+  };
+  std::vector<std::thread> pool;
+  for (unsigned w = 1; w < nw; ++w) pool.emplace_back(work, w);
+  work(0);
+  for (auto& th : pool) th.join();
+  for (size_t j = 0; j < jobs.size(); ++j) {
+    if (rc[j] != QSIM_HIP_OK) {
+      // Again on this thread, for its error message.
+      return BuildTilePass(jobs[j], mats, passes[j],
+                           recs.data() + passes[j].rec * kRec);
+    }
+  }
   return QSIM_HIP_OK;
 }
 
@@ -719,12 +764,19 @@ int qsim_hip_traj_prepare(const qsim_hip_traj_desc* trajs, unsigned num_traj,
         // The header of a pass over the gates that follow it (each checked
         // as a gate in its turn; their tile membership when the pass is
         // built).
+        // The tile holds qubits 0, 1 and 2 (the contiguous low run) and
+        // every qubit of the pass's gates.
         const uint64_t mask = uint64_t(o.q[0]) | (uint64_t(o.q[1]) << 32);
         const unsigned nt = unsigned(__builtin_popcountll(mask));
         ok = o.k >= 1 && o.k <= kBlockMaxOps && i + o.k < d[t].op_end &&
-             (mask >> n) == 0 && nt >= kBlockMinBits && nt <= kBlockMaxBits;
-        for (unsigned j = 1; ok && j <= o.k; ++j)
-          ok = ops[i + j].kind == QSIM_HIP_TRAJ_GATE;
+             (mask >> n) == 0 && nt >= kBlockMinBits && nt <= kBlockMaxBits &&
+             (mask & 7u) == 7u;
+        for (unsigned j = 1; ok && j <= o.k; ++j) {
+          const qsim_hip_traj_op& g = ops[i + j];
+          ok = g.kind == QSIM_HIP_TRAJ_GATE && g.k >= 1 && g.k <= 4;
+          for (unsigned b = 0; ok && b < g.k; ++b)
+            ok = g.q[b] < 64 && ((mask >> g.q[b]) & 1u);
+        }
       } else {
         ok = false;
       }
@@ -748,15 +800,15 @@ int qsim_hip_traj_prepare(const qsim_hip_traj_desc* trajs, unsigned num_traj,
   std::vector<TrajTilePass> passes;
   std::vector<uint16_t> recs;
   {
+    std::vector<TilePassJob> jobs;
     std::unordered_map<std::string, uint32_t> seen;
     for (unsigned t = 0; t < num_traj; ++t) {
       for (uint64_t i = d[t].op_begin; i < d[t].op_end; ++i) {
         if (ov[i].kind != QSIM_HIP_TRAJ_TILE_PASS) continue;
-        const int rc = BuildTilePass(d[t].num_qubits, ops + i, matrices,
-                                     passes, recs, seen, &ov[i].moff);
-        if (rc != QSIM_HIP_OK) return rc;
+        ov[i].moff = TilePassIndex(d[t].num_qubits, ops + i, jobs, seen);
       }
     }
+    QHIP_TRY(BuildTilePasses(jobs, matrices, passes, recs));
     if (!passes.empty()) QHIP_TRY(TileKernelReady());
   }
   auto* b = new TrajBatch;
