@@ -56,9 +56,31 @@ struct BlockPlannerParameter {
   unsigned tile_bits = 13;  // tile qubits per pass (9..14)
   unsigned low_bits = 5;    // lowest qubits always in the tile (>= 3)
   unsigned max_ops = 32;    // fused gates per pass
+  // Greedy restarts: trial 0 is the plain greedy plan; trials 1.. scale each
+  // candidate's score by a pseudo-random factor in 1 +- jitter / 2 (seeded
+  // by `seed` and the trial), and the plan with the least
+  // passes + op_cost * fused gates is kept (on MI355X a blocked pass costs
+  // about 7.75 ms of streaming plus 2.25 ms per fused gate at 32 qubits).
+  unsigned restarts = 1;
+  double jitter = 0.3;
+  double op_cost = 0.29;
+  uint64_t seed = 0;
 };
 
 namespace detail {
+
+// Pseudo-random score factors for a restart (splitmix64).
+struct Jitter {
+  uint64_t state;
+  double amount;
+  double Next() {
+    uint64_t z = (state += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    z ^= z >> 31;
+    return 1.0 + amount * (double(z >> 11) * 0x1.0p-53 - 0.5);
+  }
+};
 
 template <typename FP>
 const Gate<FP>* BlockableGate(const Operation<FP>& op) {
@@ -143,7 +165,8 @@ class Frontier {
   // Chooses a tile set (low qubits + greedy additions) and takes the gates it
   // unlocks.  Returns them in circuit order (ascending index).
   std::vector<size_t> NextPass(unsigned tile_bits, unsigned low_bits,
-                               std::vector<unsigned>& tile) {
+                               std::vector<unsigned>& tile,
+                               Jitter* jitter = nullptr) {
     const unsigned n = per_q_.size();
     std::vector<char> in(n, 0);
     unsigned size = 0;
@@ -175,7 +198,8 @@ class Frontier {
         // The closure is a unique fixpoint and grows with the qubit set, so
         // the candidate's extra gates are the closure continued from p0.
         std::vector<size_t> p1 = p0;
-        const double gain = double(ClosureFrom(in, p1, c)) / double(c.size());
+        double gain = double(ClosureFrom(in, p1, c)) / double(c.size());
+        if (jitter != nullptr) gain *= jitter->Next();
         for (unsigned x : c) in[x] = 0;
         if (gain > best) {
           best = gain;
@@ -241,8 +265,8 @@ inline std::vector<double> Embed(const std::vector<double>& m,
 // whatever its size (<= 4 qubits), so gates are merged while the union of
 // their qubits stays within 4: gate b joins an earlier gate a when no gate
 // between them touches b's qubits (b commutes back to a), as the product
-// U_b U_a on the union.
-inline void MergeTileGates(std::vector<TileGate>& g) {
+// U_b U_a on the union.  dry: qubit sets only (no matrices), for counting.
+inline void MergeTileGates(std::vector<TileGate>& g, bool dry = false) {
   auto overlaps = [](const std::vector<unsigned>& x,
                      const std::vector<unsigned>& y) {
     for (unsigned a : x)
@@ -264,6 +288,12 @@ inline void MergeTileGates(std::vector<TileGate>& g) {
           blocked = overlaps(g[k].qubits, g[b].qubits);
         if (blocked) continue;
         std::sort(u.begin(), u.end());
+        if (dry) {
+          g[a].qubits = u;
+          g.erase(g.begin() + b);
+          changed = true;
+          break;
+        }
         const auto ma = Embed(g[a].matrix, g[a].qubits, u);
         const auto mb = Embed(g[b].matrix, g[b].qubits, u);
         const unsigned D = 1u << u.size();
@@ -292,15 +322,16 @@ inline void MergeTileGates(std::vector<TileGate>& g) {
 
 }  // namespace detail
 
-// Plans blocked passes for ops of a circuit on `num_qubits` qubits.  Returns
-// false if the circuit is too small for blocked passes (num_qubits < 9).
-// `fill`, if given, supplies each fused gate's matrix (qsim's fuser then
-// skips its own products): a caller planning many similar gate lists can
-// reuse matrices it has already formed.
+namespace detail {
+
+// One greedy plan (with `jitter`, a restart's perturbed one).  dry: the
+// plan's structure only (tiles and fused-gate qubits, no matrices), for
+// comparing restarts.
 template <typename IO, typename FP>
-bool PlanBlocks(const BlockPlannerParameter& param, unsigned num_qubits,
-                const std::vector<Operation<FP>>& ops, BlockPlan& plan,
-                const std::function<void(FusedGate<FP>&)>* fill = nullptr) {
+bool PlanBlocksOnce(const BlockPlannerParameter& param, unsigned num_qubits,
+                    const std::vector<Operation<FP>>& ops, BlockPlan& plan,
+                    const std::function<void(FusedGate<FP>&)>* fill,
+                    Jitter* jitter, bool dry) {
   plan = BlockPlan{};
   const unsigned tb = std::min(param.tile_bits, num_qubits);
   if (tb < 9 || tb > 14) return false;
@@ -310,15 +341,15 @@ bool PlanBlocks(const BlockPlannerParameter& param, unsigned num_qubits,
   while (i < ops.size()) {
     // A segment of blockable gates [i, j).
     size_t j = i;
-    while (j < ops.size() && detail::BlockableGate(ops[j]) != nullptr) ++j;
+    while (j < ops.size() && BlockableGate(ops[j]) != nullptr) ++j;
     if (j > i) {
       std::vector<std::vector<unsigned>> gq;
       for (size_t t = i; t < j; ++t)
-        gq.push_back(detail::BlockableGate(ops[t])->qubits);
-      detail::Frontier fr(num_qubits, gq);
+        gq.push_back(BlockableGate(ops[t])->qubits);
+      Frontier fr(num_qubits, gq);
       while (!fr.Done()) {
         std::vector<unsigned> tile;
-        std::vector<size_t> taken = fr.NextPass(tb, lc, tile);
+        std::vector<size_t> taken = fr.NextPass(tb, lc, tile, jitter);
         if (taken.empty()) return false;  // cannot happen for k <= 4 gates
         std::vector<const Operation<FP>*> ptrs;
         for (size_t t : taken) ptrs.push_back(&ops[i + t]);
@@ -329,17 +360,17 @@ bool PlanBlocks(const BlockPlannerParameter& param, unsigned num_qubits,
         auto fused =
             MultiQubitGateFuser<IO>::template FuseGates<const Operation<FP>*>(
                 fp, num_qubits, ptrs.cbegin(), ptrs.cend(), no_splits,
-                fill == nullptr);
-        std::vector<detail::TileGate> tg;
+                !dry && fill == nullptr);
+        std::vector<TileGate> tg;
         for (auto& f : fused) {
           auto* fg = std::get_if<FusedGate<FP>>(&f);
           if (fg == nullptr) return false;  // only gates were passed in
-          if (fill != nullptr) (*fill)(*fg);
+          if (!dry && fill != nullptr) (*fill)(*fg);
           tg.push_back({fg->qubits,
                         std::vector<double>(fg->matrix.begin(),
                                             fg->matrix.end())});
         }
-        detail::MergeTileGates(tg);
+        MergeTileGates(tg, dry);
         BlockPass bp;
         bp.tile = tile;
         bp.num_gates = taken.size();
@@ -365,6 +396,48 @@ bool PlanBlocks(const BlockPlannerParameter& param, unsigned num_qubits,
     i = j + 1;
   }
   return true;
+}
+
+}  // namespace detail
+
+// Plans blocked passes for ops of a circuit on `num_qubits` qubits.  Returns
+// false if the circuit is too small for blocked passes (num_qubits < 9).
+// `fill`, if given, supplies each fused gate's matrix (qsim's fuser then
+// skips its own products): a caller planning many similar gate lists can
+// reuse matrices it has already formed.  With param.restarts > 1 the
+// greedy plan is compared with perturbed ones by their structure (passes,
+// fused gates) and the best is planned in full.
+template <typename IO, typename FP>
+bool PlanBlocks(const BlockPlannerParameter& param, unsigned num_qubits,
+                const std::vector<Operation<FP>>& ops, BlockPlan& plan,
+                const std::function<void(FusedGate<FP>&)>* fill = nullptr) {
+  if (param.restarts <= 1) {
+    return detail::PlanBlocksOnce<IO>(param, num_qubits, ops, plan, fill,
+                                      nullptr, false);
+  }
+  auto jitter_of = [&](unsigned trial) {
+    return detail::Jitter{param.seed * 0x100000001b3ull + trial, param.jitter};
+  };
+  double best = 0;
+  unsigned best_trial = 0;
+  for (unsigned trial = 0; trial < param.restarts; ++trial) {
+    detail::Jitter jt = jitter_of(trial);
+    BlockPlan p;
+    if (!detail::PlanBlocksOnce<IO>(param, num_qubits, ops, p, fill,
+                                    trial == 0 ? nullptr : &jt, true)) {
+      return false;
+    }
+    size_t nops = 0;
+    for (const auto& b : p.blocks) nops += b.ks.size();
+    const double cost = double(p.blocks.size()) + param.op_cost * double(nops);
+    if (trial == 0 || cost < best) {
+      best = cost;
+      best_trial = trial;
+    }
+  }
+  detail::Jitter jt = jitter_of(best_trial);
+  return detail::PlanBlocksOnce<IO>(param, num_qubits, ops, plan, fill,
+                                    best_trial == 0 ? nullptr : &jt, false);
 }
 
 }  // namespace hip

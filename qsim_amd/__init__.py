@@ -24,6 +24,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libqsim_hip.so")
 
 F32 = 0
 F64 = 1
+BLOCK_ACCURATE = 0x100  # QSIM_HIP_BLOCK_ACCURATE, OR-ed into a matrix precision
 # Status codes of include/qsim_hip.h.
 OK = 0
 ERR_ARG = -1
@@ -446,13 +447,14 @@ class PreparedBlock:
     """A cache-blocked pass whose fragments live on the device
     (qsim_hip_block_prepare / _run / _free)."""
 
-    def __init__(self, num_qubits, tile_qubits, gates):
+    def __init__(self, num_qubits, tile_qubits, gates, accurate=False):
         a = _block_args(tile_qubits, gates)
         tarr, tp, karr, kp, qarr, qp, mat = a
         h = ctypes.c_void_p()
+        mprec = F64 | (BLOCK_ACCURATE if accurate else 0)
         _check(lib().qsim_hip_block_prepare(num_qubits, tp, len(tarr),
                                             len(karr), kp, qp,
-                                            mat.ctypes.data, F64,
+                                            mat.ctypes.data, mprec,
                                             ctypes.byref(h)))
         self.h = h
         self.num_qubits = num_qubits
@@ -491,9 +493,11 @@ class Simulator:
             s.prec, s.ptr, s.num_qubits, qp, len(qarr), cp, len(carr), cvals,
             m.ctypes.data, mprec))
 
-    def apply_gate_block(self, tile_qubits, gates, s: State):
+    def apply_gate_block(self, tile_qubits, gates, s: State, accurate=False):
         """Applies `gates` [(qs, matrix), ...] (k <= 4, qubits inside
-        `tile_qubits`) in one cache-blocked pass; FP64 or FP32 matrices."""
+        `tile_qubits`) in one cache-blocked pass; FP64 or FP32 matrices.
+        accurate: amplitudes split in three bf16 parts (12 MFMAs per 32
+        groups instead of 8; QSIM_HIP_BLOCK_ACCURATE)."""
         tarr, tp = _u32(sorted(tile_qubits))
         ks, qs, ms = [], [], []
         for q, m in gates:
@@ -507,11 +511,12 @@ class Simulator:
                else np.zeros(2, np.float64))
         _check(lib().qsim_hip_apply_gate_block(
             s.prec, s.ptr, s.num_qubits, tp, len(tarr), len(ks), kp, qp,
-            mat.ctypes.data, F64))
+            mat.ctypes.data, F64 | (BLOCK_ACCURATE if accurate else 0)))
 
-    def prepare_gate_block(self, num_qubits, tile_qubits, gates):
+    def prepare_gate_block(self, num_qubits, tile_qubits, gates,
+                           accurate=False):
         """A blocked pass prepared once (qsim_hip_block_prepare)."""
-        return PreparedBlock(num_qubits, tile_qubits, gates)
+        return PreparedBlock(num_qubits, tile_qubits, gates, accurate)
 
     def expectation_value(self, qs, matrix, s: State) -> complex:
         qarr, qp = _u32(qs)

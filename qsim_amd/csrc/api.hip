@@ -252,6 +252,10 @@ int BlockMatrices(unsigned num_gates, const unsigned* ks, const void* matrices,
              num_gates);
     return QSIM_HIP_ERR_ARG;
   }
+  if (matrix_prec != QSIM_HIP_F32 && matrix_prec != QSIM_HIP_F64) {
+    SetError("blocked-pass matrices must be QSIM_HIP_F32 or QSIM_HIP_F64");
+    return QSIM_HIP_ERR_ARG;
+  }
   moff.resize(num_gates);
   size_t total = 0;
   for (unsigned i = 0; i < num_gates; ++i) {
@@ -636,13 +640,14 @@ int qsim_hip_apply_gate_block(int prec, void* s, unsigned n,
     return QSIM_HIP_ERR_UNSUPPORTED;
   }
   if (num_gates == 0) return QSIM_HIP_OK;
+  const bool accurate = (matrix_prec & QSIM_HIP_BLOCK_ACCURATE) != 0;
   std::vector<size_t> moff;
   std::vector<double> m64;
-  QHIP_TRY(BlockMatrices(num_gates, gate_num_qubits, matrices, matrix_prec,
-                         moff, m64));
+  QHIP_TRY(BlockMatrices(num_gates, gate_num_qubits, matrices,
+                         matrix_prec & ~QSIM_HIP_BLOCK_ACCURATE, moff, m64));
   return LaunchGateBlock(static_cast<float2*>(s), n, tile_qubits,
                          num_tile_qubits, num_gates, gate_num_qubits,
-                         gate_qubits, m64.data(), moff.data());
+                         gate_qubits, m64.data(), moff.data(), accurate);
 }
 
 int qsim_hip_block_prepare(unsigned n, const unsigned* tile_qubits,
@@ -658,13 +663,14 @@ int qsim_hip_block_prepare(unsigned n, const unsigned* tile_qubits,
   }
   *handle = nullptr;
   QHIP_TRY(EnsureInit());
+  const bool accurate = (matrix_prec & QSIM_HIP_BLOCK_ACCURATE) != 0;
   std::vector<size_t> moff;
   std::vector<double> m64;
-  QHIP_TRY(BlockMatrices(num_gates, gate_num_qubits, matrices, matrix_prec,
-                         moff, m64));
+  QHIP_TRY(BlockMatrices(num_gates, gate_num_qubits, matrices,
+                         matrix_prec & ~QSIM_HIP_BLOCK_ACCURATE, moff, m64));
   return PrepareBlockHandle(n, tile_qubits, num_tile_qubits, num_gates,
                             gate_num_qubits, gate_qubits, m64.data(),
-                            moff.data(), handle);
+                            moff.data(), handle, accurate);
 }
 
 int qsim_hip_block_run(void* handle, int prec, void* s, unsigned n) {

@@ -55,8 +55,9 @@ using namespace tileop;
 // op sequence is straight-line per program length so that the compiler's
 // wait counts stay exact.
 // TB: the kernel's tile shape; FULL: the tile holds 2^TB amplitudes, every
-// thread's 32 elements (else fewer, 2^tb with tb < TB).
-template <int TB, bool FULL>
+// thread's 32 elements (else fewer, 2^tb with tb < TB); ACC: the accurate
+// op product (ApplyTileOp).
+template <int TB, bool FULL, bool ACC>
 __global__ void __launch_bounds__(BlockShape<TB>::NT)
 __attribute__((amdgpu_waves_per_eu(2)))
 GateBlockKernel(float2* __restrict__ s, BlockGeom g, BlockProgram p,
@@ -103,7 +104,8 @@ GateBlockKernel(float2* __restrict__ s, BlockGeom g, BlockProgram p,
         rd[q] = tab[q] ^ sj;
         wr[q] = tab[8 + q] ^ sj;
       }
-      ApplyTileOp(p.bpos[op], f, rd, wr, nbatch, wave, BlockShape<TB>::kWaves);
+      ApplyTileOp<ACC>(p.bpos[op], f, rd, wr, nbatch, wave,
+                       BlockShape<TB>::kWaves);
       uint4* slot = ring4 + ((k + 1) & 1u) * (kOpWords / 4);
       slot[threadIdx.x] = pre0;
       if (kRingQuads > 1) slot[threadIdx.x + NT] = pre1;
@@ -411,6 +413,36 @@ int PrepareGateBlock(unsigned n, const unsigned* tile, unsigned nt,
   return QSIM_HIP_OK;
 }
 
+namespace {
+
+// The kernel shape for a pass's tile size.
+template <bool ACC>
+void LaunchShapes(float2* s, const BlockGeom& g, const BlockProgram& p,
+                  const uint16_t* dfrag) {
+  if (g.tb == 14) {
+    // One workgroup per CU fits in LDS (144 KiB); a long tile loop each.
+    const uint64_t blocks = std::min<uint64_t>(g.ntiles, uint64_t(NumCUs()) * 4);
+    hipLaunchKernelGGL((GateBlockKernel<14, true, ACC>), dim3(unsigned(blocks)),
+                       dim3(BlockShape<14>::NT), BlockLds<14>(), Stream(), s,
+                       g, p, dfrag);
+  } else {
+    // Two workgroups per CU fit in LDS (2 x 80 KiB); a few tiles each.
+    const uint64_t blocks =
+        std::min<uint64_t>(g.ntiles, uint64_t(NumCUs()) * 2 * 8);
+    if (g.tb == 13) {
+      hipLaunchKernelGGL((GateBlockKernel<13, true, ACC>), dim3(unsigned(blocks)),
+                         dim3(BlockShape<13>::NT), BlockLds<13>(), Stream(), s,
+                         g, p, dfrag);
+    } else {
+      hipLaunchKernelGGL((GateBlockKernel<13, false, ACC>),
+                         dim3(unsigned(blocks)), dim3(BlockShape<13>::NT),
+                         BlockLds<13>(), Stream(), s, g, p, dfrag);
+    }
+  }
+}
+
+}  // namespace
+
 int RunGateBlock(float2* s, const BlockGeom& g, const BlockProgram& p,
                  const uint16_t* dfrag) {
   // More than 64 KiB of dynamic LDS must be allowed once per process, and
@@ -423,10 +455,13 @@ int RunGateBlock(float2* s, const BlockGeom& g, const BlockProgram& p,
     const struct {
       const void* fn;
       size_t lds;
-    } fns[3] = {
-        {reinterpret_cast<const void*>(&GateBlockKernel<13, true>), BlockLds<13>()},
-        {reinterpret_cast<const void*>(&GateBlockKernel<13, false>), BlockLds<13>()},
-        {reinterpret_cast<const void*>(&GateBlockKernel<14, true>), BlockLds<14>()}};
+    } fns[6] = {
+        {reinterpret_cast<const void*>(&GateBlockKernel<13, true, false>), BlockLds<13>()},
+        {reinterpret_cast<const void*>(&GateBlockKernel<13, false, false>), BlockLds<13>()},
+        {reinterpret_cast<const void*>(&GateBlockKernel<14, true, false>), BlockLds<14>()},
+        {reinterpret_cast<const void*>(&GateBlockKernel<13, true, true>), BlockLds<13>()},
+        {reinterpret_cast<const void*>(&GateBlockKernel<13, false, true>), BlockLds<13>()},
+        {reinterpret_cast<const void*>(&GateBlockKernel<14, true, true>), BlockLds<14>()}};
     for (const auto& f : fns) {
       hipFuncAttributes fa{};
       if (attr == hipSuccess) attr = hipFuncGetAttributes(&fa, f.fn);
@@ -443,25 +478,10 @@ int RunGateBlock(float2* s, const BlockGeom& g, const BlockProgram& p,
     return QSIM_HIP_ERR_RUNTIME;
   }
   PassTimerBegin();
-  if (g.tb == 14) {
-    // One workgroup per CU fits in LDS (144 KiB); a long tile loop each.
-    const uint64_t blocks = std::min<uint64_t>(g.ntiles, uint64_t(NumCUs()) * 4);
-    hipLaunchKernelGGL((GateBlockKernel<14, true>), dim3(unsigned(blocks)),
-                       dim3(BlockShape<14>::NT), BlockLds<14>(), Stream(), s,
-                       g, p, dfrag);
+  if (p.accurate) {
+    LaunchShapes<true>(s, g, p, dfrag);
   } else {
-    // Two workgroups per CU fit in LDS (2 x 80 KiB); a few tiles each.
-    const uint64_t blocks =
-        std::min<uint64_t>(g.ntiles, uint64_t(NumCUs()) * 2 * 8);
-    if (g.tb == 13) {
-      hipLaunchKernelGGL((GateBlockKernel<13, true>), dim3(unsigned(blocks)),
-                         dim3(BlockShape<13>::NT), BlockLds<13>(), Stream(), s,
-                         g, p, dfrag);
-    } else {
-      hipLaunchKernelGGL((GateBlockKernel<13, false>), dim3(unsigned(blocks)),
-                         dim3(BlockShape<13>::NT), BlockLds<13>(), Stream(), s,
-                         g, p, dfrag);
-    }
+    LaunchShapes<false>(s, g, p, dfrag);
   }
   PassTimerEnd();
   QHIP_LAUNCHED("GateBlockKernel");
@@ -470,11 +490,12 @@ int RunGateBlock(float2* s, const BlockGeom& g, const BlockProgram& p,
 
 int LaunchGateBlock(float2* s, unsigned n, const unsigned* tile, unsigned nt,
                     unsigned nops, const unsigned* ks, const unsigned* qs,
-                    const double* m64, const size_t* moff) {
+                    const double* m64, const size_t* moff, bool accurate) {
   BlockGeom g{};
   BlockProgram p{};
   std::vector<uint16_t> fb;
   QHIP_TRY(PrepareGateBlock(n, tile, nt, nops, ks, qs, m64, moff, g, p, fb));
+  p.accurate = accurate ? 1u : 0u;
   const size_t fbytes = fb.size() * sizeof(uint16_t);
   void* dfrag = MatrixBuffer(fbytes);
   if (dfrag == nullptr) return QSIM_HIP_ERR_OOM;
@@ -492,11 +513,13 @@ struct PreparedBlock {
 
 int PrepareBlockHandle(unsigned n, const unsigned* tile, unsigned nt,
                        unsigned nops, const unsigned* ks, const unsigned* qs,
-                       const double* m64, const size_t* moff, void** out) {
+                       const double* m64, const size_t* moff, void** out,
+                       bool accurate) {
   auto* b = new PreparedBlock{};
   std::vector<uint16_t> fb;
   int rc = PrepareGateBlock(n, tile, nt, nops, ks, qs, m64, moff, b->g, b->p,
                             fb);
+  b->p.accurate = accurate ? 1u : 0u;
   if (rc != QSIM_HIP_OK) {
     delete b;
     return rc;

@@ -117,6 +117,48 @@ __device__ __forceinline__ v16f MfmaChain(const v8bf (&f)[6],
   return c;
 }
 
+// Lane half h's 8 members of its group in three bf16 parts (hi, mid, lo):
+// the amplitudes to about 2^-25 instead of 2^-18.
+__device__ __forceinline__ void SplitBf16x3(const v2f (&v)[8], v8bf (&bh)[2],
+                                            v8bf (&bm)[2], v8bf (&bl)[2]) {
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const v2bf hi = __builtin_convertvector(v[q], v2bf);
+    const v2f r1 = v[q] - __builtin_convertvector(hi, v2f);
+    const v2bf mid = __builtin_convertvector(r1, v2bf);
+    const v2f r2 = r1 - __builtin_convertvector(mid, v2f);
+    const v2bf lo = __builtin_convertvector(r2, v2bf);
+    bh[q >> 2][2 * (q & 3)] = hi[0];
+    bh[q >> 2][2 * (q & 3) + 1] = hi[1];
+    bm[q >> 2][2 * (q & 3)] = mid[0];
+    bm[q >> 2][2 * (q & 3) + 1] = mid[1];
+    bl[q >> 2][2 * (q & 3)] = lo[0];
+    bl[q >> 2][2 * (q & 3) + 1] = lo[1];
+  }
+}
+
+// The accurate product: every term down to 2^-16 relative (Al Bh + Am Bm +
+// Ah Bl + Am Bh + Ah Bm + Ah Bh per K block, smallest first), 12 MFMAs.
+__device__ __forceinline__ v16f MfmaChain6(const v8bf (&f)[6],
+                                           const v8bf (&bh)[2],
+                                           const v8bf (&bm)[2],
+                                           const v8bf (&bl)[2]) {
+  v16f c = v16f{};
+#pragma unroll
+  for (int kb = 0; kb < 2; ++kb) {
+    const v8bf& ah = f[3 * kb];
+    const v8bf& am = f[3 * kb + 1];
+    const v8bf& al = f[3 * kb + 2];
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh[kb], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(am, bm[kb], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl[kb], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(am, bh[kb], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bm[kb], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh[kb], c, 0, 0, 0);
+  }
+  return c;
+}
+
 // Lane l holds (re, im) of members 4 s + 2 h + {0, 1}.
 __device__ __forceinline__ void WriteBatch(unsigned sp, const uint32_t (&wr)[8],
                                            const v16f& c) {
@@ -128,6 +170,11 @@ __device__ __forceinline__ void WriteBatch(unsigned sp, const uint32_t (&wr)[8],
     }
 }
 
+// ACCURATE: amplitudes in three parts and the 12-MFMA product (one batch at
+// a time), for passes on structured states such as the first passes from a
+// basis state, where the two-part split's rounding is the same for many
+// amplitudes and shifts the norm coherently.
+template <bool ACCURATE = false>
 __device__ __forceinline__ void ApplyTileOp(const uint8_t* bp,
                                             const v8bf (&f)[6],
                                             const uint32_t (&rd)[8],
@@ -135,6 +182,18 @@ __device__ __forceinline__ void ApplyTileOp(const uint8_t* bp,
                                             unsigned nbatch, unsigned wave,
                                             unsigned nwaves) {
   const unsigned b[5] = {bp[0], bp[1], bp[2], bp[3], bp[4]};
+  if (ACCURATE) {
+    for (unsigned bt = wave; bt < nbatch; bt += nwaves) {
+      const unsigned sp0 = BatchSlot(bt, b);
+      v2f v0[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v0[q] = *LdsAt(sp0 ^ rd[q]);
+      v8bf bh0[2], bm0[2], bl0[2];
+      SplitBf16x3(v0, bh0, bm0, bl0);
+      WriteBatch(sp0, wr, MfmaChain6(f, bh0, bm0, bl0));
+    }
+    return;
+  }
   // Two batches at a time (their groups are disjoint): batch 1's reads and
   // split overlap batch 0's MFMA chain.
   unsigned bt = wave;

@@ -127,6 +127,7 @@ struct BlockGeom {
 
 struct BlockProgram {
   unsigned nops;
+  unsigned accurate;  // 1: amplitudes in three bf16 parts (ApplyTileOp)
   uint8_t tpos[kBlockMaxOps][4];  // tile-local targets of each op, ascending
   // Tile-local positions that carry the batch index of each op's groups
   // (tb - 9 of them; unused entries are 0 and only ever see zero bits).
@@ -197,7 +198,8 @@ void BuildFragBf16(int K, const double* m, uint16_t* frag);
 // tile) with the FP64 matrix at m64 + moff[i].
 int LaunchGateBlock(float2* s, unsigned n, const unsigned* tile, unsigned nt,
                     unsigned nops, const unsigned* ks, const unsigned* qs,
-                    const double* m64, const size_t* moff);
+                    const double* m64, const size_t* moff,
+                    bool accurate = false);
 
 // The host half of a blocked pass: validates the program and builds its
 // geometry, per-op tables and op records (kOpWords 32-bit words each, in
@@ -210,7 +212,8 @@ int PrepareGateBlock(unsigned n, const unsigned* tile, unsigned nt,
 // The same pass prepared once (device fragments kept) and run many times.
 int PrepareBlockHandle(unsigned n, const unsigned* tile, unsigned nt,
                        unsigned nops, const unsigned* ks, const unsigned* qs,
-                       const double* m64, const size_t* moff, void** out);
+                       const double* m64, const size_t* moff, void** out,
+                       bool accurate = false);
 int RunBlockHandle(void* h, float2* s, unsigned n);
 int FreeBlockHandle(void* h);
 

@@ -168,3 +168,51 @@ def test_prepared_block_runs_repeatedly(sim):
         pb.run(other)
     assert e.value.code == -1
     pb.free()
+
+
+def test_accurate_product_on_a_structured_state(sim):
+    # The first pass from |0>: every amplitude of the 16-qubit state after
+    # one layer of identical single-qubit gates has the same magnitude, so
+    # the two-part amplitude split rounds them all alike and the norm moves
+    # coherently; the three-part split (QSIM_HIP_BLOCK_ACCURATE) keeps it
+    # within FP32 rounding.  Both stay within the blocked-pass bar.
+    rng = np.random.default_rng(21)
+    n = 16
+    tile = list(range(5)) + [7, 9, 10, 12, 13, 14, 15, 11, 6]
+    tile = sorted(tile)
+    h = np.array([[1, 1], [1, -1]]) / np.sqrt(2)
+    gates = [([q], to_interleaved(h.astype(complex), np.float64))
+             for q in tile]
+    gates += random_program(tile, 20, rng, kmax=2)
+    s0 = np.zeros(2 << n)
+    s0[0] = 1.0
+    ref = s0.copy()
+    for qs, m in gates[:32]:
+        oracle.apply_gate(n, ref, qs, m)
+    drift = {}
+    for accurate in (False, True):
+        st = upload(s0.astype(np.float32), n)
+        sim.apply_gate_block(tile, gates[:32], st, accurate=accurate)
+        got = st.to_numpy().astype(np.float64)
+        drift[accurate] = abs(np.sum(got * got) - 1.0)
+        assert rel_l2(got, ref) < 1e-4
+    print(f"norm drift: two-part {drift[False]:.2e}, three-part "
+          f"{drift[True]:.2e}", flush=True)
+    assert drift[True] < 1e-6
+
+
+def test_accurate_prepared_pass_matches_oracle(sim):
+    rng = np.random.default_rng(22)
+    n = 18
+    tile = sorted(list(range(5)) + [6, 9, 10, 13, 15, 16, 17, 12])
+    gates = random_program(tile, 24, rng)
+    s = random_state(n, rng, np.float64)
+    st = upload(s.astype(np.float32), n)
+    pb = sim.prepare_gate_block(n, tile, gates, accurate=True)
+    pb.run(st)
+    pb.free()
+    for qs, m in gates:
+        oracle.apply_gate(n, s, qs, m)
+    # Three-part amplitudes: FP32-level agreement, well inside the
+    # two-part bar of 1e-4.
+    assert rel_l2(st.to_numpy(), s) < 2e-6
