@@ -107,8 +107,8 @@ inline std::vector<PlanCandidate> DefaultCandidates() {
     if (k >= 4) c.push_back({0, k, 0, QSIM_HIP_METHOD_VECTOR});
     if (k >= 5) c.push_back({0, k, 0, QSIM_HIP_METHOD_MFMA_BF16X3});
   }
-  const unsigned shapes[][2] = {{14, 5}, {14, 6}, {13, 5}, {13, 6},
-                                {13, 7}, {12, 5}, {11, 4}};
+  const unsigned shapes[][2] = {{14, 4}, {14, 5}, {14, 6}, {13, 4},
+                                {13, 5}, {13, 6}, {12, 5}, {11, 4}};
   for (const auto& s : shapes) c.push_back({1, s[0], s[1], QSIM_HIP_METHOD_AUTO});
   return c;
 }

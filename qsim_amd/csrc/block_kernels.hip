@@ -39,6 +39,20 @@ namespace {
 
 using namespace tileop;
 
+// First tile of workgroup b in a tile-stride loop of stride G.  Tiles that
+// differ only in the lowest non-tile qubits hold neighbouring memory runs
+// (with a short contiguous run, parts of the same cache lines and DRAM
+// pages); the dispatcher places workgroup b on XCD b mod 8, so each group
+// of kTileGroup such tiles goes to workgroups b, b + 8, b + 16, ... (one
+// XCD, one L2) instead of b, b + 1, ...  A bijection on [0, G) when
+// 8 kTileGroup divides G.
+constexpr unsigned kTileGroup = 4;
+__device__ __forceinline__ uint64_t PairedTile(unsigned b, unsigned G) {
+  if (G % (8u * kTileGroup) != 0) return b;
+  const unsigned q = b / (8u * kTileGroup), r = b % (8u * kTileGroup);
+  return kTileGroup * (8u * q + (r & 7u)) + (r >> 3);
+}
+
 // Each workgroup has its tile and a two-slot ring of op records in LDS:
 // while op i runs from slot i % 2, each thread holds its share (32 or 16
 // bytes) of op i + 1's record in flight from global memory (L2) and writes
@@ -113,7 +127,7 @@ GateBlockKernel(float2* __restrict__ s, BlockGeom g, BlockProgram p,
       ++k;
   };
   float2 buf[PER];
-  uint64_t tile = blockIdx.x;
+  uint64_t tile = PairedTile(blockIdx.x, gridDim.x);
   if (tile < g.ntiles) {
     io.template Fetch<FULL, 15u>(
         s, InsertZeros(tile << g.Lc, g.outer, g.nouter), g, buf);
