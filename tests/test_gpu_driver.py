@@ -34,8 +34,8 @@ def test_grid20_fused_vs_oracle(grid20, k):
     got = st.to_numpy()
     assert rel_l2(got, s32) < 1e-5
     # qsim's FP32 gate matrices are unitary only to ~3e-8 (sqrt(W) holds
-    # float(1/sqrt 2)), so the FP32-matrix state loses norm: judge its
-    # direction by the normalised fidelity.
+    # float(1/sqrt 2)), so the FP32-matrix state loses norm: its direction
+    # is measured by the normalised fidelity.
     g, f = from_interleaved(got), from_interleaved(s64)
     fid_raw = abs(np.vdot(f, g)) ** 2
     fid_norm = fid_raw / (np.vdot(g, g).real * np.vdot(f, f).real)
