@@ -50,6 +50,7 @@ struct BlockStep {
 struct BlockPlan {
   std::vector<BlockPass> blocks;
   std::vector<BlockStep> steps;
+  unsigned trial = 0;  // the restart this plan comes from (0: greedy)
 };
 
 struct BlockPlannerParameter {
@@ -65,6 +66,9 @@ struct BlockPlannerParameter {
   double jitter = 0.3;
   double op_cost = 0.29;
   uint64_t seed = 0;
+  // >= 0: plan exactly this restart (as an earlier search chose it, e.g.
+  // for a second copy of the same circuit) instead of searching.
+  int trial = -1;
 };
 
 namespace detail {
@@ -411,13 +415,20 @@ template <typename IO, typename FP>
 bool PlanBlocks(const BlockPlannerParameter& param, unsigned num_qubits,
                 const std::vector<Operation<FP>>& ops, BlockPlan& plan,
                 const std::function<void(FusedGate<FP>&)>* fill = nullptr) {
-  if (param.restarts <= 1) {
-    return detail::PlanBlocksOnce<IO>(param, num_qubits, ops, plan, fill,
-                                      nullptr, false);
-  }
   auto jitter_of = [&](unsigned trial) {
     return detail::Jitter{param.seed * 0x100000001b3ull + trial, param.jitter};
   };
+  if (param.trial > 0) {
+    detail::Jitter jt = jitter_of(unsigned(param.trial));
+    const bool ok = detail::PlanBlocksOnce<IO>(param, num_qubits, ops, plan,
+                                               fill, &jt, false);
+    plan.trial = unsigned(param.trial);
+    return ok;
+  }
+  if (param.restarts <= 1 || param.trial == 0) {
+    return detail::PlanBlocksOnce<IO>(param, num_qubits, ops, plan, fill,
+                                      nullptr, false);
+  }
   double best = 0;
   unsigned best_trial = 0;
   for (unsigned trial = 0; trial < param.restarts; ++trial) {
@@ -436,8 +447,11 @@ bool PlanBlocks(const BlockPlannerParameter& param, unsigned num_qubits,
     }
   }
   detail::Jitter jt = jitter_of(best_trial);
-  return detail::PlanBlocksOnce<IO>(param, num_qubits, ops, plan, fill,
-                                    best_trial == 0 ? nullptr : &jt, false);
+  const bool ok = detail::PlanBlocksOnce<IO>(
+      param, num_qubits, ops, plan, fill, best_trial == 0 ? nullptr : &jt,
+      false);
+  plan.trial = best_trial;
+  return ok;
 }
 
 }  // namespace hip

@@ -235,21 +235,25 @@ int RunF64(DriverCircuit& d, double* p, unsigned first, unsigned count,
   return QSIM_HIP_OK;
 }
 
+// *trial: in, the planner restart to use (-1: search); out, the one used.
 template <typename FP>
 int PlanBlocked(const Circuit<Operation<FP>>& c, unsigned tile_bits,
                 unsigned low_bits, std::vector<hip::BlockPass>& blocks,
-                std::vector<Step>& steps, FusedOps<FP>& single) {
+                std::vector<Step>& steps, FusedOps<FP>& single, int* trial) {
   hip::BlockPlannerParameter bp;
   bp.tile_bits = tile_bits;
   bp.low_bits = low_bits;
-  // 32 greedy restarts: on the 32-qubit grid circuit 23 passes instead of 26
-  // (14-qubit tiles) and 28 instead of 32 (13-qubit tiles), for about 60 ms
-  // of planning per shape.
-  bp.restarts = 32;
+  // 64 greedy restarts with scores scaled in 1 +- 0.25: on the 32-qubit grid
+  // circuit 21 passes instead of 26 for 14 tile qubits with a 4-qubit low
+  // run, for about 0.12 s of planning per shape.
+  bp.restarts = 64;
+  bp.jitter = 0.5;
+  bp.trial = *trial;
   hip::BlockPlan plan;
   if (!hip::PlanBlocks<IO>(bp, c.num_qubits, c.ops, plan)) {
     return QSIM_HIP_ERR_UNSUPPORTED;
   }
+  *trial = int(plan.trial);
   blocks = std::move(plan.blocks);
   steps.clear();
   single.clear();
@@ -380,10 +384,13 @@ int qsim_driver_fuse_blocked(void* c, unsigned tile_bits, unsigned low_bits) {
   d->ReleasePrepared();
   std::vector<hip::BlockPass> b32;
   std::vector<Step> s32;
+  // The FP32 copy of the circuit has the same structure: it takes the
+  // restart the FP64 search chose.
+  int trial = -1;
   int rc = PlanBlocked(d->c64, tile_bits, low_bits, d->blocks, d->steps,
-                       d->f64);
+                       d->f64, &trial);
   if (rc == QSIM_HIP_OK) {
-    rc = PlanBlocked(d->c32, tile_bits, low_bits, b32, s32, d->f32);
+    rc = PlanBlocked(d->c32, tile_bits, low_bits, b32, s32, d->f32, &trial);
   }
   if (rc == QSIM_HIP_OK && s32.size() != d->steps.size()) {
     rc = QSIM_HIP_ERR_RUNTIME;
