@@ -25,7 +25,7 @@ TILED = 1
 # Autotuner candidates: (threads, plan); plan is a fusion size or a tile
 # spec ("tile", tile_bits, low_bits).
 DEFAULT_CANDIDATES = ((TILED, ("tile", 14, 5)), (TILED, ("tile", 14, 4)),
-                      (TILED, ("tile", 13, 5)), (TILED, ("tile", 13, 4)),
+                      (TILED, ("tile", 14, 3)), (TILED, ("tile", 13, 4)),
                       (TILED, ("tile", 12, 5)), (256, 4), (512, 4),
                       (1024, 4), (1024, 3))
 
