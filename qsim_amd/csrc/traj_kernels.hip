@@ -27,9 +27,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <mutex>
+#include <new>
 #include <string>
+#include <system_error>
 #include <thread>
 #include <type_traits>
 #include <unordered_map>
@@ -645,8 +648,11 @@ int BuildTilePass(const TilePassJob& job, const float* mats, TrajTilePass& tp,
   return rc;
 }
 
-// Builds the distinct passes, spread over a few host threads (each pass is
-// independent; the lane-position memo is shared under its lock).
+// Builds the distinct passes, spread over a few host threads that take jobs
+// from a shared counter (each pass is independent; the lane-position memo is
+// shared under its lock).  A thread that cannot be started leaves its share
+// to the others, and a failed host allocation in a worker becomes an error
+// return instead of ending the process.
 int BuildTilePasses(const std::vector<TilePassJob>& jobs, const float* mats,
                     std::vector<TrajTilePass>& passes,
                     std::vector<uint16_t>& recs) {
@@ -658,21 +664,38 @@ int BuildTilePasses(const std::vector<TilePassJob>& jobs, const float* mats,
     total += jobs[j].o[0].k;
   }
   recs.assign(total * kRec, 0);
+  constexpr int kHostAlloc = 1;  // worker-side marker, never returned
   std::vector<int> rc(jobs.size(), QSIM_HIP_OK);
   const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
   const unsigned nw = unsigned(std::min<size_t>(
       std::min(hw, 16u), std::max<size_t>(1, jobs.size() / 64)));
-  auto work = [&](unsigned w) {
-    for (size_t j = w; j < jobs.size(); j += nw) {
-      rc[j] = BuildTilePass(jobs[j], mats, passes[j],
-                            recs.data() + passes[j].rec * kRec);
+  std::atomic<size_t> next{0};
+  auto work = [&]() {
+    for (size_t j; (j = next.fetch_add(1)) < jobs.size();) {
+      try {
+        rc[j] = BuildTilePass(jobs[j], mats, passes[j],
+                              recs.data() + passes[j].rec * kRec);
+      } catch (const std::bad_alloc&) {
+        rc[j] = kHostAlloc;
+      }
     }
   };
   std::vector<std::thread> pool;
-  for (unsigned w = 1; w < nw; ++w) pool.emplace_back(work, w);
-  work(0);
+  pool.reserve(nw);
+  for (unsigned w = 1; w < nw; ++w) {
+    try {
+      pool.emplace_back(work);
+    } catch (const std::system_error&) {
+      break;
+    }
+  }
+  work();
   for (auto& th : pool) th.join();
   for (size_t j = 0; j < jobs.size(); ++j) {
+    if (rc[j] == kHostAlloc) {
+      SetError("traj_prepare: out of host memory building tile pass %zu", j);
+      return QSIM_HIP_ERR_RUNTIME;
+    }
     if (rc[j] != QSIM_HIP_OK) {
       // Again on this thread, for its error message.
       return BuildTilePass(jobs[j], mats, passes[j],
