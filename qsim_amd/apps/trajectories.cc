@@ -23,11 +23,13 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <new>
 #include <random>
 #include <sstream>
 #include <string>
@@ -395,12 +397,23 @@ int PlanBatch(TrajBatchHost* b, const unsigned* circuit_of,
   b->have_results = false;
   b->plans.assign(num_traj, TrajPlan{});
   std::vector<FusedMatrixCache> caches(omp_get_max_threads());
+  // An exception may not leave the parallel region: a failed host
+  // allocation is caught per trajectory and returned as an error.
+  std::atomic<bool> out_of_memory{false};
 #pragma omp parallel for schedule(dynamic, 4)
   for (int t = 0; t < int(num_traj); ++t) {
     b->plans[t].circuit = circuit_of[t];
     b->plans[t].rep = reps[t];
-    PlanTrajectory(b->circuits[circuit_of[t]], reps[t], spec,
-                   caches[omp_get_thread_num()], b->plans[t]);
+    try {
+      PlanTrajectory(b->circuits[circuit_of[t]], reps[t], spec,
+                     caches[omp_get_thread_num()], b->plans[t]);
+    } catch (const std::bad_alloc&) {
+      out_of_memory = true;
+    }
+  }
+  if (out_of_memory) {
+    b->plans.clear();
+    return QSIM_HIP_ERR_RUNTIME;
   }
   // Device order: largest states first.
   b->order.resize(num_traj);
