@@ -58,15 +58,16 @@ METRIC = ("32q circuit time + HBM GB/s, learned/MFMA vs plain FP32 fusion "
           "at F>=1-1e-6")
 
 # name -> (partition, max_fused_size or tile qubits, FP64 fused matrices or
-# low tile qubits, gate method)
+# low tile qubits, gate method, blocked-pass arithmetic)
 PATHS = {
-    "plain-fp32-vector": ("mqubit", 4, False, 1),
-    "fp32-mfma": ("mqubit", 4, False, 0),
-    "bf16-k6": ("mqubit", 6, True, 2),
-    "blocked-13": ("blocked", 13, 5, 0),
-    "blocked-14": ("blocked", 14, 5, 0),
+    "plain-fp32-vector": ("mqubit", 4, False, 1, "f16"),
+    "fp32-mfma": ("mqubit", 4, False, 0, "f16"),
+    "bf16-k6": ("mqubit", 6, True, 2, "f16"),
+    "blocked-13": ("blocked", 13, 5, 0, "f16"),
+    "blocked-14": ("blocked", 14, 4, 0, "f16"),
+    "blocked-14-bf16": ("blocked", 14, 4, 0, "bf16"),
     # the learned planner's pick (qsim_amd/data/cost_model.json)
-    "learned": ("learned", 0, 0, 0),
+    "learned": ("learned", 0, 0, 0, "f16"),
 }
 
 # HBM bytes per amplitude per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE
@@ -92,7 +93,8 @@ def fuse(circ, name):
     planner pick or None)."""
     global _MODEL
     from qsim_amd import costmodel, driver
-    part, a, b, method = PATHS[name]
+    part, a, b, method, arith = PATHS[name]
+    circ.set_block_arith(arith)
     if part == "learned":
         if _MODEL is None:
             _MODEL = costmodel.CostModel.load()
@@ -128,7 +130,7 @@ def run_path(args, text, name):
     import qsim_amd
     from qsim_amd import driver
 
-    part, kmax, f64m, _ = PATHS[name]
+    part, kmax, f64m, _, _ = PATHS[name]
     circ = driver.Circuit(text)
     t0 = time.time()
     npass, method, pick = fuse(circ, name)

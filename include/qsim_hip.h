@@ -208,22 +208,29 @@ int qsim_hip_apply_controlled_gate(int prec, void* state, unsigned num_qubits,
                                    int matrix_prec);
 
 /* A cache-blocked pass: applies num_gates (<= 32) gates in order with one
- * stream of the state through HBM.  The state is cut into tiles over the
- * tile qubits (ascending, 9..14 of them, starting with the run 0, 1, 2, ...
- * of at least 3 qubits, at most 11 beyond it); gate i acts on
- * gate_num_qubits[i] (1..4) ascending qubits gate_qubits[4 i ...], all tile
- * qubits, with the next 2 * 4^k matrix elements of `matrices` (row-major
- * interleaved, bit j <-> qubit j; matrix_prec QSIM_HIP_F32 or QSIM_HIP_F64,
- * optionally | QSIM_HIP_BLOCK_ACCURATE).  Arithmetic: split-BF16 MFMA
- * products with FP32 accumulation, the matrix in 3 bf16 parts and the state
- * in 2 (or 3 with QSIM_HIP_BLOCK_ACCURATE: 12 MFMAs per 32 groups instead of
- * 8, for passes on structured states -- the first passes from a basis
- * state -- where the 2-part split's rounding repeats across amplitudes and
- * moves the norm coherently).  Complex64 states only (else
- * QSIM_HIP_ERR_UNSUPPORTED).  Equivalent to one SimulatorCUDA::ApplyGate
- * (lib/simulator_cuda.h:70-125) per gate, in the apply loop of
- * QSimRunner::Run (lib/run_qsim.h:264-280). */
-enum { QSIM_HIP_BLOCK_ACCURATE = 0x100 };
+ * stream of the state through HBM (programs longer than the kernel's LDS
+ * holds, 14 ops for 14 tile qubits and 7 otherwise, stream once per such
+ * chunk).  The state is cut into tiles over the tile qubits (ascending,
+ * 9..14 of them, starting with the run 0, 1, 2, ... of at least 3 qubits, at
+ * most 11 beyond it); gate i acts on gate_num_qubits[i] (1..4) ascending
+ * qubits gate_qubits[4 i ...], all tile qubits, with the next 2 * 4^k matrix
+ * elements of `matrices` (row-major interleaved, bit j <-> qubit j;
+ * matrix_prec QSIM_HIP_F32 or QSIM_HIP_F64, optionally OR-ed with the flags
+ * below).  Arithmetic, MFMA products with FP32 accumulation:
+ *  - default: FP16 with scaling -- each tile is scaled by a power of two
+ *    when staged, matrix and amplitudes are split in 2 fp16 parts each, 3
+ *    products (6 MFMAs per 32 groups); about 2^-22 relative per gate.
+ *    Programs whose matrices could grow the state by more than 2^10 run on
+ *    split-BF16 instead;
+ *  - QSIM_HIP_BLOCK_BF16: split-BF16, the matrix in 3 bf16 parts and the
+ *    state in 2 (8 MFMAs, about 2^-17 per gate), or 3 with
+ *    QSIM_HIP_BLOCK_ACCURATE (12 MFMAs; for passes on structured states --
+ *    the first passes from a basis state -- where the 2-part split's
+ *    rounding repeats across amplitudes and moves the norm coherently).
+ * Complex64 states only (else QSIM_HIP_ERR_UNSUPPORTED).  Equivalent to one
+ * SimulatorCUDA::ApplyGate (lib/simulator_cuda.h:70-125) per gate, in the
+ * apply loop of QSimRunner::Run (lib/run_qsim.h:264-280). */
+enum { QSIM_HIP_BLOCK_ACCURATE = 0x100, QSIM_HIP_BLOCK_BF16 = 0x200 };
 int qsim_hip_apply_gate_block(int prec, void* state, unsigned num_qubits,
                               const unsigned* tile_qubits,
                               unsigned num_tile_qubits, unsigned num_gates,

@@ -73,6 +73,12 @@ int qsim_driver_block_info(void* circuit, unsigned i, unsigned* num_ops,
  * the library's current one); set for the run only and restored after. */
 int qsim_driver_set_method(void* circuit, int method);
 
+/* Arithmetic of the circuit's blocked passes: FP16 with per-tile scaling
+ * (the default) or split-BF16 (three-part amplitudes in the circuit's first
+ * blocked pass); see qsim_hip_apply_gate_block. */
+enum { QSIM_DRIVER_BLOCK_F16 = 0, QSIM_DRIVER_BLOCK_BF16 = 1 };
+int qsim_driver_set_block_arith(void* circuit, int arith);
+
 /* Cost-model features (include/qsim_amd/learned_planner.h, 18 floats) of
  * pass i of the current partition when run with gate method `method`. */
 int qsim_driver_pass_features(void* circuit, unsigned i, int method,

@@ -25,6 +25,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libqsim_hip.so")
 F32 = 0
 F64 = 1
 BLOCK_ACCURATE = 0x100  # QSIM_HIP_BLOCK_ACCURATE, OR-ed into a matrix precision
+BLOCK_BF16 = 0x200  # QSIM_HIP_BLOCK_BF16: split-BF16 instead of scaled FP16
 # Status codes of include/qsim_hip.h.
 OK = 0
 ERR_ARG = -1
@@ -443,15 +444,23 @@ def _block_args(tile_qubits, gates):
     return (tarr, tp, karr, kp, qarr, qp, mat)
 
 
+def _block_flags(accurate, arith):
+    if arith not in ("f16", "bf16"):
+        raise ValueError(f"blocked-pass arithmetic {arith!r}: 'f16' or 'bf16'")
+    return (F64 | (BLOCK_ACCURATE if accurate else 0)
+            | (BLOCK_BF16 if arith == "bf16" else 0))
+
+
 class PreparedBlock:
     """A cache-blocked pass whose fragments live on the device
     (qsim_hip_block_prepare / _run / _free)."""
 
-    def __init__(self, num_qubits, tile_qubits, gates, accurate=False):
+    def __init__(self, num_qubits, tile_qubits, gates, accurate=False,
+                 arith="f16"):
         a = _block_args(tile_qubits, gates)
         tarr, tp, karr, kp, qarr, qp, mat = a
         h = ctypes.c_void_p()
-        mprec = F64 | (BLOCK_ACCURATE if accurate else 0)
+        mprec = _block_flags(accurate, arith)
         _check(lib().qsim_hip_block_prepare(num_qubits, tp, len(tarr),
                                             len(karr), kp, qp,
                                             mat.ctypes.data, mprec,
@@ -493,11 +502,14 @@ class Simulator:
             s.prec, s.ptr, s.num_qubits, qp, len(qarr), cp, len(carr), cvals,
             m.ctypes.data, mprec))
 
-    def apply_gate_block(self, tile_qubits, gates, s: State, accurate=False):
+    def apply_gate_block(self, tile_qubits, gates, s: State, accurate=False,
+                         arith="f16"):
         """Applies `gates` [(qs, matrix), ...] (k <= 4, qubits inside
         `tile_qubits`) in one cache-blocked pass; FP64 or FP32 matrices.
-        accurate: amplitudes split in three bf16 parts (12 MFMAs per 32
-        groups instead of 8; QSIM_HIP_BLOCK_ACCURATE)."""
+        arith: "f16" (scaled two-part FP16 products, the default) or "bf16"
+        (QSIM_HIP_BLOCK_BF16: split-BF16); accurate: with "bf16",
+        amplitudes split in three bf16 parts (12 MFMAs per 32 groups instead
+        of 8; QSIM_HIP_BLOCK_ACCURATE)."""
         tarr, tp = _u32(sorted(tile_qubits))
         ks, qs, ms = [], [], []
         for q, m in gates:
@@ -511,12 +523,12 @@ class Simulator:
                else np.zeros(2, np.float64))
         _check(lib().qsim_hip_apply_gate_block(
             s.prec, s.ptr, s.num_qubits, tp, len(tarr), len(ks), kp, qp,
-            mat.ctypes.data, F64 | (BLOCK_ACCURATE if accurate else 0)))
+            mat.ctypes.data, _block_flags(accurate, arith)))
 
     def prepare_gate_block(self, num_qubits, tile_qubits, gates,
-                           accurate=False):
+                           accurate=False, arith="f16"):
         """A blocked pass prepared once (qsim_hip_block_prepare)."""
-        return PreparedBlock(num_qubits, tile_qubits, gates, accurate)
+        return PreparedBlock(num_qubits, tile_qubits, gates, accurate, arith)
 
     def expectation_value(self, qs, matrix, s: State) -> complex:
         qarr, qp = _u32(qs)

@@ -54,6 +54,8 @@ struct DriverCircuit {
   std::vector<Step> steps;
   // Gate method the partition runs with (-1: the library's current one).
   int method = -1;
+  // Arithmetic of the blocked passes (QSIM_DRIVER_BLOCK_*).
+  int block_arith = QSIM_DRIVER_BLOCK_F16;
   // Set while qsim_driver_capture records and captures the passes.
   bool capturing = false;
   // Prepared blocked passes (qsim_hip_block_prepare), one per block, made at
@@ -120,11 +122,14 @@ void PassInfo(const FusedOp& op, unsigned* k, unsigned* qubits,
 // Blocked pass `index`: prepared (fragments on the device) at its first run,
 // one kernel launch per run after that.
 int RunBlock(DriverCircuit& d, size_t index, float* p, unsigned n) {
-  // The circuit's first blocked pass runs on a structured state (a basis
-  // state and its first gates), where the two-part amplitude split would
-  // move the norm coherently: it uses the accurate product.
+  // Split-BF16: the circuit's first blocked pass runs on a structured state
+  // (a basis state and its first gates), where the two-part amplitude split
+  // would move the norm coherently, so it uses the accurate product.
   const int mprec =
-      QSIM_HIP_F64 | (index == 0 ? int(QSIM_HIP_BLOCK_ACCURATE) : 0);
+      d.block_arith == QSIM_DRIVER_BLOCK_BF16
+          ? QSIM_HIP_F64 | int(QSIM_HIP_BLOCK_BF16) |
+                (index == 0 ? int(QSIM_HIP_BLOCK_ACCURATE) : 0)
+          : QSIM_HIP_F64;
   if (d.capturing) {
     // A captured graph owns its data: the fragments go through the
     // library's staging (the graph plan's buffers), not the circuit's
@@ -487,6 +492,16 @@ int qsim_driver_set_method(void* c, int method) {
     return QSIM_HIP_ERR_ARG;
   }
   d->method = method;
+  return QSIM_HIP_OK;
+}
+
+int qsim_driver_set_block_arith(void* c, int arith) {
+  auto* d = static_cast<DriverCircuit*>(c);
+  if (arith != QSIM_DRIVER_BLOCK_F16 && arith != QSIM_DRIVER_BLOCK_BF16) {
+    return QSIM_HIP_ERR_ARG;
+  }
+  if (arith != d->block_arith) d->ReleasePrepared();
+  d->block_arith = arith;
   return QSIM_HIP_OK;
 }
 

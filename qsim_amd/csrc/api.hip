@@ -641,13 +641,19 @@ int qsim_hip_apply_gate_block(int prec, void* s, unsigned n,
   }
   if (num_gates == 0) return QSIM_HIP_OK;
   const bool accurate = (matrix_prec & QSIM_HIP_BLOCK_ACCURATE) != 0;
+  const int arith = (matrix_prec & QSIM_HIP_BLOCK_BF16) != 0
+                        ? int(kBlockArithBf16)
+                        : int(kBlockArithF16);
   std::vector<size_t> moff;
   std::vector<double> m64;
-  QHIP_TRY(BlockMatrices(num_gates, gate_num_qubits, matrices,
-                         matrix_prec & ~QSIM_HIP_BLOCK_ACCURATE, moff, m64));
+  QHIP_TRY(BlockMatrices(
+      num_gates, gate_num_qubits, matrices,
+      matrix_prec & ~(QSIM_HIP_BLOCK_ACCURATE | QSIM_HIP_BLOCK_BF16), moff,
+      m64));
   return LaunchGateBlock(static_cast<float2*>(s), n, tile_qubits,
                          num_tile_qubits, num_gates, gate_num_qubits,
-                         gate_qubits, m64.data(), moff.data(), accurate);
+                         gate_qubits, m64.data(), moff.data(), accurate,
+                         arith);
 }
 
 int qsim_hip_block_prepare(unsigned n, const unsigned* tile_qubits,
@@ -664,13 +670,18 @@ int qsim_hip_block_prepare(unsigned n, const unsigned* tile_qubits,
   *handle = nullptr;
   QHIP_TRY(EnsureInit());
   const bool accurate = (matrix_prec & QSIM_HIP_BLOCK_ACCURATE) != 0;
+  const int arith = (matrix_prec & QSIM_HIP_BLOCK_BF16) != 0
+                        ? int(kBlockArithBf16)
+                        : int(kBlockArithF16);
   std::vector<size_t> moff;
   std::vector<double> m64;
-  QHIP_TRY(BlockMatrices(num_gates, gate_num_qubits, matrices,
-                         matrix_prec & ~QSIM_HIP_BLOCK_ACCURATE, moff, m64));
+  QHIP_TRY(BlockMatrices(
+      num_gates, gate_num_qubits, matrices,
+      matrix_prec & ~(QSIM_HIP_BLOCK_ACCURATE | QSIM_HIP_BLOCK_BF16), moff,
+      m64));
   return PrepareBlockHandle(n, tile_qubits, num_tile_qubits, num_gates,
                             gate_num_qubits, gate_qubits, m64.data(),
-                            moff.data(), handle, accurate);
+                            moff.data(), handle, accurate, arith);
 }
 
 int qsim_hip_block_run(void* handle, int prec, void* s, unsigned n) {

@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cmath>
 #include <cstring>
 #include <mutex>
 #include <type_traits>
@@ -181,6 +182,78 @@ GateBlockKernel(float2* __restrict__ s, BlockGeom g, BlockProgram p,
   }
 }
 
+// The FP16-with-scaling pass (block_tile.h).  Every op record of the pass
+// is copied to LDS once per workgroup, so the ops issue no global loads:
+// the only vector memory operations are the tile's loads and stores, and
+// the wait for the next tile's loads (issued in full at the start of the
+// current tile's ops) never stands behind a record fetch issued after the
+// previous tile's stores.  The stores of a tile drain while the next
+// tile's ops run.  The tile is scaled by 2^e when staged and by 2^-e when
+// stored (TileScale, exact); one more barrier per tile for the maximum.
+template <int TB, bool FULL>
+__global__ void __launch_bounds__(BlockShape<TB>::NT)
+__attribute__((amdgpu_waves_per_eu(2)))
+GateBlockF16Kernel(float2* __restrict__ s, BlockGeom g, BlockProgram p,
+                   const uint32_t* __restrict__ recs) {
+  constexpr int NT = BlockShape<TB>::NT;
+  constexpr int NW = BlockShape<TB>::kWaves;
+  constexpr int PER = (1 << TB) / NT;
+  static_assert(PER == 32, "tile elements per thread");
+  extern __shared__ char lds_raw[];
+  float2* lds = reinterpret_cast<float2*>(lds_raw);
+  float* slots = reinterpret_cast<float*>(lds_raw + F16Lds<TB>::kMaxOff);
+  uint32_t* rec = reinterpret_cast<uint32_t*>(lds_raw + F16Lds<TB>::kRecOff);
+  const int l = threadIdx.x & 63;
+  const unsigned wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const unsigned nbatch = (1u << g.tb) >> 9;  // 32 groups of 16 per batch
+  const unsigned nper = (1u << g.tb) / NT;
+  {
+    const unsigned nq = p.nops * unsigned(kF16RecWords / 4);
+    const uint4* src = reinterpret_cast<const uint4*>(recs);
+    uint4* dst = reinterpret_cast<uint4*>(rec);
+    for (unsigned q = threadIdx.x; q < nq; q += NT) dst[q] = src[q];
+  }
+  TileIo<PER, NT> io;
+  io.Init(g);
+  float2 buf[PER];
+  uint64_t tile = PairedTile(blockIdx.x, gridDim.x);
+  int e = 0;
+  if (tile < g.ntiles) {
+    io.template Fetch<FULL, 15u>(
+        s, InsertZeros(tile << g.Lc, g.outer, g.nouter), g, buf);
+    e = TileScale<NW>(LocalMax<PER, FULL>(buf, nper), slots, p.headroom);
+    io.template PutScaled<FULL>(g, buf, lds, e);
+  }
+  __syncthreads();
+  for (; tile < g.ntiles; tile += gridDim.x) {
+    const uint64_t base = InsertZeros(tile << g.Lc, g.outer, g.nouter);
+    // The next tile of this workgroup, or this one again past the end (an
+    // in-bounds read that is never used), so the prefetch is unconditional.
+    const uint64_t next = tile + gridDim.x < g.ntiles ? tile + gridDim.x : tile;
+    io.template Fetch<FULL, 15u>(
+        s, InsertZeros(next << g.Lc, g.outer, g.nouter), g, buf);
+    for (unsigned op = 0; op < p.nops; ++op) {
+      const uint32_t* r = rec + op * unsigned(kF16RecWords);
+      v8h f[4];
+      LoadFragF16(r, l, f);
+      const uint32_t* tab = r + kF16TabOff + 16 * (l >> 5);
+      const uint32_t sj = r[kF16TabOff + kTableWords + (l & 31)];
+      uint32_t rd[8], wr[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        rd[q] = tab[q] ^ sj;
+        wr[q] = tab[8 + q] ^ sj;
+      }
+      ApplyTileOpF16(p.bpos[op], f, rd, wr, nbatch, wave, NW);
+      __syncthreads();
+    }
+    io.template StoreScaled<16, FULL>(s, base, g, lds, e);
+    e = TileScale<NW>(LocalMax<PER, FULL>(buf, nper), slots, p.headroom);
+    io.template PutScaled<FULL>(g, buf, lds, e);
+    __syncthreads();
+  }
+}
+
 // TileSwz on the host (the slot of tile-local amplitude a).
 unsigned HostTileSwz(unsigned a) { return a ^ (((a >> 4) ^ (a >> 8)) & 15u); }
 
@@ -301,12 +374,75 @@ void PadTo4(const double* u, unsigned k, const unsigned* pos,
   }
 }
 
+// Round-to-nearest-even double -> IEEE fp16 bits (overflow to infinity).
+uint16_t HalfBits(double x) {
+  const uint16_t sign = std::signbit(x) ? 0x8000u : 0u;
+  const double a = std::fabs(x);
+  if (a == 0.0) return sign;
+  if (!(a < 65520.0)) return uint16_t(sign | 0x7c00u);
+  int e;
+  (void)std::frexp(a, &e);  // a in [2^(e-1), 2^e)
+  if (e - 1 < -14) {        // subnormal: multiples of 2^-24
+    return uint16_t(sign | unsigned(std::nearbyint(std::ldexp(a, 24))));
+  }
+  double m = std::nearbyint(std::ldexp(a, 11 - e));  // [2^10, 2^11]
+  int E = e - 1;
+  if (m == 2048.0) {
+    m = 1024.0;
+    ++E;
+  }
+  if (E > 15) return uint16_t(sign | 0x7c00u);
+  return uint16_t(sign | unsigned((E + 15) << 10) | (unsigned(m) - 1024u));
+}
+
+double HalfValue(uint16_t b) {
+  const int E = (b >> 10) & 31, m = b & 1023;
+  const double v = E == 0 ? std::ldexp(double(m), -24)
+                          : std::ldexp(double(1024 + m), E - 25);
+  return (b & 0x8000u) ? -v : v;
+}
+
+// Upper bound of the spectral norm of a 16 x 16 complex matrix: the square
+// root of the largest absolute row sum of U^dagger U (which bounds its
+// largest eigenvalue); 1 up to rounding for a unitary matrix.
+double NormBound16(const double* u) {
+  double best = 0.0;
+  for (unsigned i = 0; i < 16; ++i) {
+    double row = 0.0;
+    for (unsigned j = 0; j < 16; ++j) {
+      double re = 0.0, im = 0.0;
+      for (unsigned r = 0; r < 16; ++r) {
+        const double ar = u[2 * (16 * r + i)], ai = -u[2 * (16 * r + i) + 1];
+        const double br = u[2 * (16 * r + j)], bi = u[2 * (16 * r + j) + 1];
+        re += ar * br - ai * bi;
+        im += ar * bi + ai * br;
+      }
+      row += std::hypot(re, im);
+    }
+    best = std::max(best, row);
+  }
+  return std::sqrt(best);
+}
+
+// FP16 op record of the padded matrix u (FP64) and the tables.
+void BuildRecF16(const double* u, const uint32_t* tab, uint32_t* rec) {
+  for (unsigned e = 0; e < 256; ++e) {
+    const uint16_t hr = HalfBits(u[2 * e]), hi = HalfBits(u[2 * e + 1]);
+    const uint16_t lr = HalfBits(u[2 * e] - HalfValue(hr));
+    const uint16_t li = HalfBits(u[2 * e + 1] - HalfValue(hi));
+    rec[e] = uint32_t(hr) | (uint32_t(hi) << 16);
+    rec[kF16MatWords + e] = uint32_t(lr) | (uint32_t(li) << 16);
+  }
+  std::memcpy(rec + kF16TabOff, tab,
+              sizeof(uint32_t) * (kTableWords + kLaneWords));
+}
+
 }  // namespace
 
 int PrepareGateBlock(unsigned n, const unsigned* tile, unsigned nt,
                      unsigned nops, const unsigned* ks, const unsigned* qs,
                      const double* m64, const size_t* moff, BlockGeom& g,
-                     BlockProgram& p, std::vector<uint16_t>& fb) {
+                     BlockProgram& p, std::vector<uint16_t>& fb, int arith) {
   if (nt < kBlockMinBits || nt > kBlockMaxBits || nt > n) {
     SetError("a blocked pass needs %u..%u tile qubits (at most n = %u), got %u",
              kBlockMinBits, kBlockMaxBits, n, nt);
@@ -359,9 +495,12 @@ int PrepareGateBlock(unsigned n, const unsigned* tile, unsigned nt,
 
   p = BlockProgram{};
   p.nops = nops;
-  constexpr size_t kOpElems = 2 * size_t(kOpWords);  // uint16 per op record
+  p.arith = unsigned(arith);
+  const size_t kOpElems =  // uint16 per op record
+      2 * size_t(arith == kBlockArithF16 ? kF16RecWords : kOpWords);
   fb.assign(std::max<size_t>(nops, 1) * kOpElems, 0);
   std::vector<double> u16(2 * 16 * 16);
+  double growth = 1.0, max_growth = 1.0;  // products of op norm bounds
   for (unsigned i = 0; i < nops; ++i) {
     const unsigned k = ks[i];
     if (k < 1 || k > 4) {
@@ -395,7 +534,12 @@ int PrepareGateBlock(unsigned n, const unsigned* tile, unsigned nt,
     }
     PadTo4(m64 + moff[i], k, pos, pad, u16.data());
     uint16_t* rec = fb.data() + i * kOpElems;
-    BuildFragBf16(4, u16.data(), rec);
+    if (arith == kBlockArithF16) {
+      growth *= NormBound16(u16.data());
+      max_growth = std::max(max_growth, growth);
+    } else {
+      BuildFragBf16(4, u16.data(), rec);
+    }
     // Lane tables: member m of a group sits at tile offset toff(m) (bit b of
     // m at position pad[b]); lane half h reads members 4h + 8(q/4) + q%4 and
     // writes members 2h + 4q + u (the operand maps of the MFMA product).
@@ -421,7 +565,26 @@ int PrepareGateBlock(unsigned n, const unsigned* tile, unsigned nt,
           uint32_t(sizeof(float2)) * HostTileSwz(Deposit(j, lane, 5));
     }
     for (unsigned b = 0; b < kBlockMaxBits - 9; ++b) p.bpos[i][b] = uint8_t(batch[b]);
-    std::memcpy(rec + 2 * size_t(kFragWords), tab, sizeof(tab));
+    if (arith == kBlockArithF16) {
+      uint32_t r32[kF16RecWords];
+      BuildRecF16(u16.data(), tab, r32);
+      std::memcpy(rec, r32, sizeof(r32));
+    } else {
+      std::memcpy(rec + 2 * size_t(kFragWords), tab, sizeof(tab));
+    }
+  }
+  if (arith == kBlockArithF16) {
+    // The tile's largest value is staged 2^headroom lower when the ops can
+    // grow the tile's norm (non-unitary matrices): the 2^13.5 bound of the
+    // scaled values leaves a factor 4 below fp16's 65504 (TileScale).
+    // Beyond 2^10 of growth fp16 would lose the small values: such a
+    // program runs on the split-BF16 product (FP32 exponent range).
+    const int lg = int(std::ceil(std::log2(max_growth * (1.0 + 1e-9))));
+    if (lg > 10) {
+      return PrepareGateBlock(n, tile, nt, nops, ks, qs, m64, moff, g, p, fb,
+                              kBlockArithBf16);
+    }
+    p.headroom = unsigned(std::max(0, lg - 2));
   }
 
   return QSIM_HIP_OK;
@@ -455,6 +618,42 @@ void LaunchShapes(float2* s, const BlockGeom& g, const BlockProgram& p,
   }
 }
 
+// The FP16 pass: programs longer than the shape's LDS holds run as
+// consecutive launches of at most F16Lds<TB>::kMaxOps ops each.
+void LaunchF16(float2* s, const BlockGeom& g, const BlockProgram& p,
+               const uint32_t* drec) {
+  const unsigned maxops =
+      g.tb == 14 ? F16Lds<14>::kMaxOps : F16Lds<13>::kMaxOps;
+  for (unsigned c0 = 0; c0 < p.nops; c0 += maxops) {
+    BlockProgram q = p;
+    q.nops = std::min(maxops, p.nops - c0);
+    for (unsigned i = 0; i < q.nops; ++i) {
+      std::memcpy(q.tpos[i], p.tpos[c0 + i], sizeof(q.tpos[i]));
+      std::memcpy(q.bpos[i], p.bpos[c0 + i], sizeof(q.bpos[i]));
+    }
+    const uint32_t* r = drec + size_t(c0) * kF16RecWords;
+    if (g.tb == 14) {
+      const uint64_t blocks =
+          std::min<uint64_t>(g.ntiles, uint64_t(NumCUs()) * 4);
+      hipLaunchKernelGGL((GateBlockF16Kernel<14, true>), dim3(unsigned(blocks)),
+                         dim3(BlockShape<14>::NT), F16Lds<14>::Bytes(q.nops),
+                         Stream(), s, g, q, r);
+    } else {
+      const uint64_t blocks =
+          std::min<uint64_t>(g.ntiles, uint64_t(NumCUs()) * 2 * 8);
+      if (g.tb == 13) {
+        hipLaunchKernelGGL((GateBlockF16Kernel<13, true>),
+                           dim3(unsigned(blocks)), dim3(BlockShape<13>::NT),
+                           F16Lds<13>::Bytes(q.nops), Stream(), s, g, q, r);
+      } else {
+        hipLaunchKernelGGL((GateBlockF16Kernel<13, false>),
+                           dim3(unsigned(blocks)), dim3(BlockShape<13>::NT),
+                           F16Lds<13>::Bytes(q.nops), Stream(), s, g, q, r);
+      }
+    }
+  }
+}
+
 }  // namespace
 
 int RunGateBlock(float2* s, const BlockGeom& g, const BlockProgram& p,
@@ -469,13 +668,19 @@ int RunGateBlock(float2* s, const BlockGeom& g, const BlockProgram& p,
     const struct {
       const void* fn;
       size_t lds;
-    } fns[6] = {
+    } fns[] = {
         {reinterpret_cast<const void*>(&GateBlockKernel<13, true, false>), BlockLds<13>()},
         {reinterpret_cast<const void*>(&GateBlockKernel<13, false, false>), BlockLds<13>()},
         {reinterpret_cast<const void*>(&GateBlockKernel<14, true, false>), BlockLds<14>()},
         {reinterpret_cast<const void*>(&GateBlockKernel<13, true, true>), BlockLds<13>()},
         {reinterpret_cast<const void*>(&GateBlockKernel<13, false, true>), BlockLds<13>()},
-        {reinterpret_cast<const void*>(&GateBlockKernel<14, true, true>), BlockLds<14>()}};
+        {reinterpret_cast<const void*>(&GateBlockKernel<14, true, true>), BlockLds<14>()},
+        {reinterpret_cast<const void*>(&GateBlockF16Kernel<13, true>),
+         F16Lds<13>::Bytes(F16Lds<13>::kMaxOps)},
+        {reinterpret_cast<const void*>(&GateBlockF16Kernel<13, false>),
+         F16Lds<13>::Bytes(F16Lds<13>::kMaxOps)},
+        {reinterpret_cast<const void*>(&GateBlockF16Kernel<14, true>),
+         F16Lds<14>::Bytes(F16Lds<14>::kMaxOps)}};
     for (const auto& f : fns) {
       hipFuncAttributes fa{};
       if (attr == hipSuccess) attr = hipFuncGetAttributes(&fa, f.fn);
@@ -492,7 +697,9 @@ int RunGateBlock(float2* s, const BlockGeom& g, const BlockProgram& p,
     return QSIM_HIP_ERR_RUNTIME;
   }
   PassTimerBegin();
-  if (p.accurate) {
+  if (p.arith == kBlockArithF16) {
+    LaunchF16(s, g, p, reinterpret_cast<const uint32_t*>(dfrag));
+  } else if (p.accurate) {
     LaunchShapes<true>(s, g, p, dfrag);
   } else {
     LaunchShapes<false>(s, g, p, dfrag);
@@ -504,11 +711,13 @@ int RunGateBlock(float2* s, const BlockGeom& g, const BlockProgram& p,
 
 int LaunchGateBlock(float2* s, unsigned n, const unsigned* tile, unsigned nt,
                     unsigned nops, const unsigned* ks, const unsigned* qs,
-                    const double* m64, const size_t* moff, bool accurate) {
+                    const double* m64, const size_t* moff, bool accurate,
+                    int arith) {
   BlockGeom g{};
   BlockProgram p{};
   std::vector<uint16_t> fb;
-  QHIP_TRY(PrepareGateBlock(n, tile, nt, nops, ks, qs, m64, moff, g, p, fb));
+  QHIP_TRY(
+      PrepareGateBlock(n, tile, nt, nops, ks, qs, m64, moff, g, p, fb, arith));
   p.accurate = accurate ? 1u : 0u;
   const size_t fbytes = fb.size() * sizeof(uint16_t);
   void* dfrag = MatrixBuffer(fbytes);
@@ -528,11 +737,11 @@ struct PreparedBlock {
 int PrepareBlockHandle(unsigned n, const unsigned* tile, unsigned nt,
                        unsigned nops, const unsigned* ks, const unsigned* qs,
                        const double* m64, const size_t* moff, void** out,
-                       bool accurate) {
+                       bool accurate, int arith) {
   auto* b = new PreparedBlock{};
   std::vector<uint16_t> fb;
   int rc = PrepareGateBlock(n, tile, nt, nops, ks, qs, m64, moff, b->g, b->p,
-                            fb);
+                            fb, arith);
   b->p.accurate = accurate ? 1u : 0u;
   if (rc != QSIM_HIP_OK) {
     delete b;

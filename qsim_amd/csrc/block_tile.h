@@ -224,6 +224,176 @@ __device__ __forceinline__ void ApplyTileOp(const uint8_t* bp,
   }
 }
 
+// ---- FP16-with-scaling ops ---------------------------------------------
+//
+// The same product on v_mfma_f32_32x32x16_f16: the matrix in two fp16 parts
+// (Ah + Al, 22 bits), the amplitudes in two (Bh + Bl, 22 bits) and
+// Al Bh + Ah Bl + Ah Bh accumulated in FP32 (6 MFMAs per 32 groups instead
+// of 8; the dropped Al Bl is 2^-22 relative).  fp16 has a 5-bit exponent,
+// so the tile is scaled by a power of two when it is staged (TileScale):
+// its largest real or imaginary part goes to [2^5, 2^6) / 2^headroom.  The
+// ops of a pass keep the tile's norm (unitary, or contractive for headroom
+// 0), so no value passes sqrt(2 * 2^14) * 2^6 < 2^14 < 65504; values below
+// 2^-8 of the tile's largest keep an absolute error under 2^-30 of it.
+typedef _Float16 v8h __attribute__((ext_vector_type(8)));
+typedef _Float16 v2h __attribute__((ext_vector_type(2)));
+
+// FP16 op record, 32-bit words: the 16 x 16 complex matrix as packed
+// (re, im) fp16 pairs at word 16 r + c, hi part then lo part; the member
+// tables and the group table as in the bf16 record.
+constexpr int kF16MatWords = 256;
+constexpr int kF16TabOff = 2 * kF16MatWords;
+constexpr int kF16RecWords = kF16TabOff + kTableWords + kLaneWords;  // 576
+static_assert(kF16RecWords % 4 == 0, "records are copied as uint4");
+
+// LDS of an FP16 blocked-pass workgroup: the tile, 16 floats for the
+// per-wave maxima, then the pass's op records (all of them, loaded once per
+// workgroup).  14-qubit tiles: one workgroup per CU (160 KiB); 13: two.
+template <int TB>
+struct F16Lds {
+  static constexpr size_t kLimit = TB == 14 ? 163840 : 81920;
+  static constexpr size_t kMaxOff = BlockShape<TB>::kTileBytes;
+  static constexpr size_t kRecOff = kMaxOff + 64;
+  static constexpr unsigned kMaxOps =
+      unsigned((kLimit - kRecOff) / (kF16RecWords * sizeof(uint32_t)));
+  static constexpr size_t Bytes(unsigned nops) {
+    return kRecOff + size_t(nops) * kF16RecWords * sizeof(uint32_t);
+  }
+};
+
+// Lane l's A fragments of the op whose record is at r (LDS): for K block
+// kb, f[2 kb] = hi and f[2 kb + 1] = lo of A[l%32][16 kb + 8 (l/32) + e].
+// Row l%32 = 2 i + p reads U[i][8 kb + 4 (l/32) + c], c < 4, and forms
+// (Re, -Im) for p = 0 or (Im, Re) for p = 1 from the stored (Re, Im) pair.
+__device__ __forceinline__ void LoadFragF16(const uint32_t* r, int l,
+                                            v8h (&f)[4]) {
+  const unsigned i = unsigned(l & 31) >> 1, hh = unsigned(l) >> 5;
+  const unsigned rot = (l & 1) ? 16u : 0u;
+  const uint32_t sx = (l & 1) ? 0u : 0x80000000u;
+#pragma unroll
+  for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+    for (int part = 0; part < 2; ++part) {
+      const uint4 w = *reinterpret_cast<const uint4*>(
+          r + part * kF16MatWords + 16 * i + 8 * kb + 4 * hh);
+      uint4 x;
+      x.x = __builtin_amdgcn_alignbit(w.x, w.x, rot) ^ sx;
+      x.y = __builtin_amdgcn_alignbit(w.y, w.y, rot) ^ sx;
+      x.z = __builtin_amdgcn_alignbit(w.z, w.z, rot) ^ sx;
+      x.w = __builtin_amdgcn_alignbit(w.w, w.w, rot) ^ sx;
+      f[2 * kb + part] = __builtin_bit_cast(v8h, x);
+    }
+}
+
+// Lane half h's 8 members as fp16 (hi, lo) pairs: hi rounds to nearest, the
+// residual v - hi is exact in FP32 (one v_fma_mix_f32 from the fp16 half)
+// and rounds to lo.
+__device__ __forceinline__ void SplitF16(const v2f (&v)[8], v8h (&bh)[2],
+                                         v8h (&bl)[2]) {
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const v2h hi = __builtin_convertvector(v[q], v2h);
+    float rx, ry;
+    asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]"
+        : "=v"(rx)
+        : "v"(hi), "v"(v[q][0]));
+    asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]"
+        : "=v"(ry)
+        : "v"(hi), "v"(v[q][1]));
+    const v2h lo = __builtin_convertvector((v2f{rx, ry}), v2h);
+    bh[q >> 2][2 * (q & 3)] = hi[0];
+    bh[q >> 2][2 * (q & 3) + 1] = hi[1];
+    bl[q >> 2][2 * (q & 3)] = lo[0];
+    bl[q >> 2][2 * (q & 3) + 1] = lo[1];
+  }
+}
+
+// Al Bh + Ah Bl + Ah Bh per K block, smallest first, one accumulator chain.
+__device__ __forceinline__ v16f MfmaChainF16(const v8h (&f)[4],
+                                             const v8h (&bh)[2],
+                                             const v8h (&bl)[2]) {
+  v16f c = v16f{};
+#pragma unroll
+  for (int kb = 0; kb < 2; ++kb) {
+    c = __builtin_amdgcn_mfma_f32_32x32x16_f16(f[2 * kb + 1], bh[kb], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x16_f16(f[2 * kb], bl[kb], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x16_f16(f[2 * kb], bh[kb], c, 0, 0, 0);
+  }
+  return c;
+}
+
+// One FP16 op over the (scaled) LDS tile; batches as in ApplyTileOp, two
+// at a time.
+__device__ __forceinline__ void ApplyTileOpF16(const uint8_t* bp,
+                                               const v8h (&f)[4],
+                                               const uint32_t (&rd)[8],
+                                               const uint32_t (&wr)[8],
+                                               unsigned nbatch, unsigned wave,
+                                               unsigned nwaves) {
+  const unsigned b[5] = {bp[0], bp[1], bp[2], bp[3], bp[4]};
+  unsigned bt = wave;
+  for (; bt + nwaves < nbatch; bt += 2 * nwaves) {
+    const unsigned sp0 = BatchSlot(bt, b);
+    const unsigned sp1 = BatchSlot(bt + nwaves, b);
+    v2f v0[8], v1[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v0[q] = *LdsAt(sp0 ^ rd[q]);
+    v8h bh0[2], bl0[2], bh1[2], bl1[2];
+    SplitF16(v0, bh0, bl0);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v1[q] = *LdsAt(sp1 ^ rd[q]);
+    const v16f c0 = MfmaChainF16(f, bh0, bl0);
+    SplitF16(v1, bh1, bl1);
+    const v16f c1 = MfmaChainF16(f, bh1, bl1);
+    WriteBatch(sp0, wr, c0);
+    WriteBatch(sp1, wr, c1);
+  }
+  if (bt < nbatch) {
+    const unsigned sp0 = BatchSlot(bt, b);
+    v2f v0[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v0[q] = *LdsAt(sp0 ^ rd[q]);
+    v8h bh0[2], bl0[2];
+    SplitF16(v0, bh0, bl0);
+    WriteBatch(sp0, wr, MfmaChainF16(f, bh0, bl0));
+  }
+}
+
+// Largest |re| or |im| over the first `nper` of a thread's PER elements
+// (FULL: all of them).
+template <int PER, bool FULL>
+__device__ __forceinline__ float LocalMax(const float2 (&v)[PER],
+                                          unsigned nper) {
+  float m = 0.0f;
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    if (FULL || unsigned(i) < nper) {
+      m = fmaxf(m, fmaxf(fabsf(v[i].x), fabsf(v[i].y)));
+    }
+  }
+  return m;
+}
+
+// Workgroup maximum of `m` through `slots` (one float per wave), then the
+// tile's scale exponent: the largest value to [2^5, 2^6) / 2^headroom, or
+// 0 for an all-zero (or non-finite) tile.  Ends with a barrier before the
+// slots are read; the slots are written again only after a later barrier.
+template <int NWAVES>
+__device__ __forceinline__ int TileScale(float m, float* slots,
+                                         unsigned headroom) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) m = fmaxf(m, __shfl_xor(m, d));
+  if ((threadIdx.x & 63) == 0) slots[threadIdx.x >> 6] = m;
+  __syncthreads();
+  float t = 0.0f;
+#pragma unroll
+  for (int w = 0; w < NWAVES; ++w) t = fmaxf(t, slots[w]);
+  if (!(t > 0.0f) || !(t <= 3.0e38f)) return 0;
+  int e;
+  (void)frexpf(t, &e);  // t in [2^(e-1), 2^e)
+  return 6 - int(headroom) - e;  // |e| <= 149: ldexpf takes any such shift
+}
+
 // Offset (in amplitudes) of tile-local high index x: x's bits deposited at
 // the tile's high qubits.
 __device__ __forceinline__ uint64_t DepositHigh(unsigned x, const BlockGeom& g) {
@@ -292,6 +462,46 @@ struct TileIo {
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
       if (FULL || unsigned(i) < nper) lds[Slot(i)] = v[i];
+    }
+  }
+
+  // Put with every value multiplied by 2^e (exact for normal results).
+  template <bool FULL>
+  __device__ __forceinline__ void PutScaled(const BlockGeom& g,
+                                            const float2 (&v)[PER],
+                                            float2* lds, int e) const {
+    const unsigned nper = (1u << g.tb) >> kLogNT;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      if (FULL || unsigned(i) < nper) {
+        lds[Slot(i)] = float2{ldexpf(v[i].x, e), ldexpf(v[i].y, e)};
+      }
+    }
+  }
+
+  // Store with every value multiplied by 2^-e.
+  template <int CH, bool FULL>
+  __device__ __forceinline__ void StoreScaled(float2* __restrict__ s,
+                                              uint64_t base,
+                                              const BlockGeom& g,
+                                              const float2* lds,
+                                              int e) const {
+    const unsigned nper = (1u << g.tb) >> kLogNT;
+    float2* p = s + base + tofs;
+#pragma unroll
+    for (int i0 = 0; i0 < PER; i0 += CH) {
+      float2 v[CH];
+#pragma unroll
+      for (int i = 0; i < CH; ++i) {
+        if (FULL || unsigned(i0 + i) < nper) {
+          const float2 x = lds[Slot(i0 + i)];
+          v[i] = float2{ldexpf(x.x, -e), ldexpf(x.y, -e)};
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < CH; ++i) {
+        if (FULL || unsigned(i0 + i) < nper) p[uint64_t(eofs[i0 + i]) << g.Lc] = v[i];
+      }
     }
   }
 

@@ -125,9 +125,17 @@ struct BlockGeom {
   unsigned tb;
 };
 
+// Arithmetic of a blocked pass's ops: split-BF16 (matrix in 3 parts,
+// amplitudes in 2, or 3 when `accurate`) or FP16 with per-tile scaling
+// (matrix and amplitudes in 2 parts each).
+constexpr unsigned kBlockArithBf16 = 0;
+constexpr unsigned kBlockArithF16 = 1;
+
 struct BlockProgram {
   unsigned nops;
-  unsigned accurate;  // 1: amplitudes in three bf16 parts (ApplyTileOp)
+  unsigned accurate;  // bf16: amplitudes in three parts (ApplyTileOp)
+  unsigned arith;     // kBlockArith*
+  unsigned headroom;  // f16: the tile is staged 2^headroom lower
   uint8_t tpos[kBlockMaxOps][4];  // tile-local targets of each op, ascending
   // Tile-local positions that carry the batch index of each op's groups
   // (tb - 9 of them; unused entries are 0 and only ever see zero bits).
@@ -199,21 +207,24 @@ void BuildFragBf16(int K, const double* m, uint16_t* frag);
 int LaunchGateBlock(float2* s, unsigned n, const unsigned* tile, unsigned nt,
                     unsigned nops, const unsigned* ks, const unsigned* qs,
                     const double* m64, const size_t* moff,
-                    bool accurate = false);
+                    bool accurate = false, int arith = kBlockArithF16);
 
 // The host half of a blocked pass: validates the program and builds its
-// geometry, per-op tables and op records (kOpWords 32-bit words each, in
-// `fb` as 16-bit halves).
+// geometry, per-op tables and op records (kOpWords 32-bit words each for
+// split-BF16, kF16RecWords for FP16, in `fb` as 16-bit halves).  An FP16
+// program whose matrices can grow the state by more than 2^10 is built as
+// split-BF16 instead (p.arith says which).
 int PrepareGateBlock(unsigned n, const unsigned* tile, unsigned nt,
                      unsigned nops, const unsigned* ks, const unsigned* qs,
                      const double* m64, const size_t* moff, BlockGeom& g,
-                     BlockProgram& p, std::vector<uint16_t>& fb);
+                     BlockProgram& p, std::vector<uint16_t>& fb,
+                     int arith = kBlockArithBf16);
 
 // The same pass prepared once (device fragments kept) and run many times.
 int PrepareBlockHandle(unsigned n, const unsigned* tile, unsigned nt,
                        unsigned nops, const unsigned* ks, const unsigned* qs,
                        const double* m64, const size_t* moff, void** out,
-                       bool accurate = false);
+                       bool accurate = false, int arith = kBlockArithF16);
 int RunBlockHandle(void* h, float2* s, unsigned n);
 int FreeBlockHandle(void* h);
 

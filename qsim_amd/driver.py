@@ -54,6 +54,7 @@ def driver_lib():
                 ("qsim_driver_block_info", i, [vp, u, up, up, up,
                                                ctypes.POINTER(ctypes.c_double)]),
                 ("qsim_driver_set_method", i, [vp, i]),
+                ("qsim_driver_set_block_arith", i, [vp, i]),
                 ("qsim_driver_pass_features", i, [vp, u, i, vp]),
                 ("qsim_driver_plan_learned", i, [vp, vp, u, vp, vp, vp, vp,
                                                  vp, vp, up, vp]),
@@ -96,10 +97,19 @@ class Circuit:
         return self.num_passes
 
     def fuse_blocked(self, tile_bits: int = 13, low_bits: int = 5):
-        """Cache-blocked passes (FP64 fused matrices, BF16x3 MFMA)."""
+        """Cache-blocked passes (FP64 fused matrices; MFMA products in the
+        arithmetic set_block_arith chose, scaled FP16 by default)."""
         _check(driver_lib().qsim_driver_fuse_blocked(self.h, tile_bits,
                                                      low_bits))
         return self.num_passes
+
+    def set_block_arith(self, arith: str):
+        """Blocked-pass arithmetic: "f16" (scaled two-part FP16, the
+        default) or "bf16" (split-BF16)."""
+        if arith not in ("f16", "bf16"):
+            raise ValueError(f"blocked-pass arithmetic {arith!r}")
+        _check(driver_lib().qsim_driver_set_block_arith(
+            self.h, 0 if arith == "f16" else 1))
 
     def set_method(self, method: int):
         """Gate method the partition runs with (-1: the library's)."""
