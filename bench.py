@@ -283,28 +283,58 @@ def fidelities(args, names, picks=None):
     return out
 
 
+def host_cores():
+    """Threads for the CPU baseline: every CPU this process may run on (its
+    affinity set), capped by OMP_NUM_THREADS where the host sets it (the
+    GPU host gives each GPU's jobs a share of its cores that way); and the
+    counts behind the choice."""
+    aff = len(os.sched_getaffinity(0))
+    omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(aff, omp) if omp > 0 else aff
+    return threads, {"affinity_cpus": aff, "machine_cpus": os.cpu_count(),
+                     "omp_num_threads": omp or None}
+
+
 def run_cpu_baseline(args, circuit_file):
+    """qsim's own CPU path on the same circuit file: QSimRunner::Run with
+    MultiQubitGateFuser at max_fused_size 4 (lib/run_qsim.h:225-289) on the
+    simulator lib/simmux.h selects for the host (SimulatorAVX512 where the
+    CPU has AVX-512, else SimulatorAVX), every core of the process's share.
+    The whole circuit is timed unless --cpu-passes P bounds it to the first
+    P fused passes (then scaled by the pass count)."""
     import oracle
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-    p = args.cpu_passes
-    log(f"cpu baseline: qsim SimulatorAVX, {threads} threads, first {p} passes")
+    threads, cores = host_cores()
+    if args.cpu_threads:
+        threads = args.cpu_threads
+    p = args.cpu_passes if args.cpu_passes > 0 else None
+    log(f"cpu baseline: qsim QSimRunner on the host's widest SIMD simulator, "
+        f"{threads} threads ({cores}), "
+        + (f"first {p} passes" if p else "full circuit"))
     t0 = time.time()
     info = oracle.run_reference(circuit_file, mode="avx", max_fused=4,
-                                threads=threads, passes=p, timeout=1800)
+                                threads=threads, passes=p, timeout=1800,
+                                widest_simd=True)
     wall = time.time() - t0
-    per_pass = info["sim_s"] / max(1, min(p, info["fused_ops"]))
-    per_circuit = per_pass * info["fused_ops"]
-    log(f"cpu baseline: {info['sim_s']:.2f}s for {p} passes "
-        f"({per_pass:.3f}s/pass), {info['fused_ops']} passes/circuit, "
-        f"process wall {wall:.1f}s")
+    if p:
+        per_pass = info["sim_s"] / max(1, min(p, info["fused_ops"]))
+        per_circuit = per_pass * info["fused_ops"]
+        sample = (f"first {p} of {info['fused_ops']} fused passes "
+                  f"(MultiQubitGateFuser k=4) of the same "
+                  f"{info['num_qubits']}q circuit; seconds per pass x passes "
+                  f"per circuit")
+    else:
+        per_circuit = info["sim_s"]
+        sample = (f"full circuit: QSimRunner::Run (MultiQubitGateFuser k=4, "
+                  f"{info['fused_ops']} fused passes, fusion included) of the "
+                  f"same {info['num_qubits']}q circuit file")
+    log(f"cpu baseline: {info['simulator']} {info['sim_s']:.2f}s "
+        f"({info['fused_ops']} passes/circuit), process wall {wall:.1f}s")
     return {
         "value": per_circuit, "unit": "s/circuit", "cores": threads,
         "kind": "reference",
-        "sample": (f"first {p} of {info['fused_ops']} fused passes "
-                   f"(MultiQubitGateFuser k=4) of the same "
-                   f"{info['num_qubits']}q circuit on qsim's SimulatorAVX "
-                   f"(oracle/_ref), {threads} OpenMP threads; seconds per "
-                   f"pass x passes per circuit"),
+        "sample": (f"{sample} on qsim's {info['simulator']} (oracle/_ref), "
+                   f"{threads} OpenMP threads"),
+        "simulator": info["simulator"], "host": cores,
         "sample_s": info["sim_s"], "passes_per_circuit_cpu": info["fused_ops"],
     }
 
@@ -426,7 +456,9 @@ def trajectories_cpu(args, sizes):
     batch: 4 trajectories per size, scaled to the batch's mix."""
     import oracle
     from qsim_amd import circuits
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    threads, cores = host_cores()
+    if args.cpu_threads:
+        threads = args.cpu_threads
     per = 4
     total = 0.0
     with tempfile.TemporaryDirectory() as tmp:
@@ -437,11 +469,11 @@ def trajectories_cpu(args, sizes):
             rows, cols = circuits.grid_shape(n)
             _, s = oracle.run_reference_trajectories(
                 f, rows, cols, 0, per, 1e-3, 5e-3, max_fused=4,
-                threads=threads)
+                threads=threads, widest_simd=True)
             total += s["sim_s"] / per
     per_traj = total / len(sizes)
     return {"value": 1.0 / per_traj, "unit": "trajectories/s",
-            "cores": threads, "kind": "reference",
+            "cores": threads, "host": cores, "kind": "reference",
             "sample": f"{per} trajectories per size on qsim's "
                       f"QuantumTrajectorySimulator + SimulatorAVX, "
                       f"{threads} OpenMP threads"}
@@ -460,7 +492,9 @@ def main():
                     " (default: all; 'learned' needs the trained model)")
     ap.add_argument("--fidelity-qubits", type=int, default=28)
     ap.add_argument("--no-fidelity", action="store_true")
-    ap.add_argument("--cpu-passes", type=int, default=6)
+    ap.add_argument("--cpu-passes", type=int, default=0,
+                    help="bound the CPU baseline to the first P fused passes "
+                         "(default 0: the whole circuit)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pmc-calibration", action="store_true",

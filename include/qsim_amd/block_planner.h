@@ -37,6 +37,9 @@ struct BlockPass {
   std::vector<unsigned> ks;    // per fused gate: qubit count (<= 4)
   std::vector<unsigned> qs;    // per fused gate: 4 slots, ascending qubits
   std::vector<double> m;       // fused matrices, concatenated (2 * 4^k each)
+  // per fused gate: indices (into the planned ops) of its circuit gates, in
+  // the order they act
+  std::vector<std::vector<size_t>> comps;
   unsigned num_gates = 0;      // circuit gates in the pass
 };
 
@@ -235,6 +238,7 @@ class Frontier {
 struct TileGate {
   std::vector<unsigned> qubits;
   std::vector<double> matrix;
+  std::vector<size_t> comps;  // circuit gates in it (indices), in order
 };
 
 // m over `qs` (ascending) embedded in the space of `u` (ascending superset),
@@ -292,6 +296,8 @@ inline void MergeTileGates(std::vector<TileGate>& g, bool dry = false) {
           blocked = overlaps(g[k].qubits, g[b].qubits);
         if (blocked) continue;
         std::sort(u.begin(), u.end());
+        g[a].comps.insert(g[a].comps.end(), g[b].comps.begin(),
+                          g[b].comps.end());
         if (dry) {
           g[a].qubits = u;
           g.erase(g.begin() + b);
@@ -365,14 +371,27 @@ bool PlanBlocksOnce(const BlockPlannerParameter& param, unsigned num_qubits,
             MultiQubitGateFuser<IO>::template FuseGates<const Operation<FP>*>(
                 fp, num_qubits, ptrs.cbegin(), ptrs.cend(), no_splits,
                 !dry && fill == nullptr);
+        // The circuit index of each component gate.
+        std::vector<std::pair<const Gate<FP>*, size_t>> where;
+        for (size_t t : taken) where.push_back({BlockableGate(ops[i + t]), i + t});
+        auto index_of = [&](const typename FusedGate<FP>::PGate& pg) {
+          const Gate<FP>* const* gp = std::get_if<const Gate<FP>*>(&pg);
+          for (const auto& w : where)
+            if (gp != nullptr && w.first == *gp) return w.second;
+          return size_t(-1);
+        };
         std::vector<TileGate> tg;
         for (auto& f : fused) {
           auto* fg = std::get_if<FusedGate<FP>>(&f);
           if (fg == nullptr) return false;  // only gates were passed in
           if (!dry && fill != nullptr) (*fill)(*fg);
-          tg.push_back({fg->qubits,
-                        std::vector<double>(fg->matrix.begin(),
-                                            fg->matrix.end())});
+          TileGate t{fg->qubits,
+                     std::vector<double>(fg->matrix.begin(), fg->matrix.end()),
+                     {}};
+          if (!dry) {
+            for (const auto& pg : fg->gates) t.comps.push_back(index_of(pg));
+          }
+          tg.push_back(std::move(t));
         }
         MergeTileGates(tg, dry);
         BlockPass bp;
@@ -384,14 +403,16 @@ bool PlanBlocksOnce(const BlockPlannerParameter& param, unsigned num_qubits,
           bp.ks.clear();
           bp.qs.clear();
           bp.m.clear();
+          bp.comps.clear();
           bp.num_gates = 0;
         };
-        for (const auto& t : tg) {
+        for (auto& t : tg) {
           if (bp.ks.size() == param.max_ops) flush();
           bp.ks.push_back(t.qubits.size());
           for (unsigned b = 0; b < 4; ++b)
             bp.qs.push_back(b < t.qubits.size() ? t.qubits[b] : 0);
           bp.m.insert(bp.m.end(), t.matrix.begin(), t.matrix.end());
+          bp.comps.push_back(std::move(t.comps));
         }
         if (!bp.ks.empty()) flush();
       }

@@ -251,6 +251,37 @@ int qsim_hip_block_run(void* handle, int prec, void* state,
                        unsigned num_qubits);
 int qsim_hip_block_free(void* handle);
 
+/* Gate queue.  While on (tile_bits 9..14; 0 turns it off), complex64
+ * qsim_hip_apply_gate calls on 1..4 qubits are not run one pass each but
+ * collected into cache-blocked passes (qsim_hip_apply_gate_block, FP16
+ * with per-tile scaling): consecutive gates on one state whose qubits,
+ * together with qubits 0 .. low_bits - 1, fit in tile_bits qubits form one
+ * pass (at most 14 gates for 14 tile qubits, 7 otherwise), run when the
+ * next gate does not fit or goes to another state, or at the first other
+ * call of the library -- every other entry point runs the queued gates
+ * first, so results read through the library are those of the gates run
+ * one by one, up to the blocked pass's rounding (about 2^-22 relative per
+ * gate).  A state must not be read or freed behind the library's back
+ * while gates for it are queued (qsim_hip_flush runs them).  Grouping
+ * consecutive gates greedily gives the fewest passes of any grouping into
+ * consecutive runs, so gates emitted pass by pass by a blocked-pass
+ * planner (LearnedFuserHIP) come back as those passes.  No reference
+ * counterpart: it lets qsim's unchanged apply loop (lib/run_qsim.h:264-280,
+ * lib/gate_appl.h:35-48) reach blocked passes through
+ * SimulatorCUDA::ApplyGate's interface (lib/simulator_cuda.h:70-125). */
+int qsim_hip_set_gate_queue(unsigned tile_bits, unsigned low_bits);
+/* The queue's setting and the blocked passes (and gates in them) it has run
+ * in this process. */
+int qsim_hip_get_gate_queue(unsigned* tile_bits, unsigned* low_bits,
+                            uint64_t* passes, uint64_t* gates);
+/* Runs the queued gates now. */
+int qsim_hip_flush(void);
+
+/* Gates an FP16 blocked pass over tile_bits tile qubits keeps resident (a
+ * longer program streams the state once per such chunk): 14 for 14 tile
+ * qubits, 7 below. */
+unsigned qsim_hip_block_resident_ops(unsigned tile_bits);
+
 /* <psi|U|psi> for k <= 6 with FP64 accumulation.  Replaces SimulatorCUDA::
  * ExpectationValue (lib/simulator_cuda.h:216-260). */
 int qsim_hip_expectation_value(int prec, const void* state,
@@ -399,6 +430,13 @@ int qsim_hip_costmodel_set_params(void* model, const float* params,
                                   uint64_t num_params);
 int qsim_hip_costmodel_get_params(void* model, float* params,
                                   uint64_t num_params);
+/* A model saved as JSON {"num_features": F, "hidden": H, "params": [...]}
+ * (qsim_amd/data/cost_model.json), created on the device. */
+int qsim_hip_costmodel_load(const char* path, void** model);
+/* The process's default model, which the learned fuser uses when its
+ * parameter names none (NULL: none; freeing the model clears it). */
+int qsim_hip_costmodel_set_default(void* model);
+void* qsim_hip_costmodel_default(void);
 /* y[i] = network(x[i * num_features ...]) for i < batch (host arrays). */
 int qsim_hip_costmodel_forward(void* model, const float* x, unsigned batch,
                                float* y);

@@ -16,6 +16,20 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_BIN = os.path.join(HERE, "_ref", "qsim_ref")
+REF_BIN_AVX512 = os.path.join(HERE, "_ref", "qsim_ref_avx512")
+
+
+def host_has_avx512() -> bool:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    fl = line.split()
+                    return all(x in fl for x in ("avx512f", "avx512dq",
+                                                 "avx512bw", "avx512vl"))
+    except OSError:
+        pass
+    return False
 
 _lib = None
 
@@ -265,12 +279,17 @@ class Circuit:
 
 def run_reference(circuit_file, mode="avx", max_fused=2, threads=1, seed=1,
                   state_out=None, num_samples=0, sample_seed=0, bits_out=None,
-                  passes=None, timeout=3600):
-    """Runs the compiled reference driver (oracle/_ref/qsim_ref)."""
+                  passes=None, timeout=3600, widest_simd=False):
+    """Runs the compiled reference driver (oracle/_ref/qsim_ref; with
+    widest_simd, its AVX-512 build where the host has AVX-512, so that
+    lib/simmux.h selects SimulatorAVX512 as a native build would)."""
     import json
-    if not os.path.exists(REF_BIN):
-        raise FileNotFoundError(REF_BIN)
-    cmd = [REF_BIN, "-c", circuit_file, "-m", mode, "-f", str(max_fused),
+    exe = REF_BIN
+    if widest_simd and os.path.exists(REF_BIN_AVX512) and host_has_avx512():
+        exe = REF_BIN_AVX512
+    if not os.path.exists(exe):
+        raise FileNotFoundError(exe)
+    cmd = [exe, "-c", circuit_file, "-m", mode, "-f", str(max_fused),
            "-t", str(threads), "-s", str(seed)]
     if state_out:
         cmd += ["-o", state_out]
@@ -293,14 +312,17 @@ def reference_draws(num, seed, max_value, path):
 
 def run_reference_trajectories(circuit_file, rows, cols, rep0, rep1,
                                p_depol=1e-3, gamma=5e-3, max_fused=4,
-                               threads=1, timeout=3600):
+                               threads=1, timeout=3600, widest_simd=False):
     """qsim's QuantumTrajectorySimulator on the noisy grid circuit
     (oracle/_ref/qsim_ref -T): per repetition the Kraus index of every
     channel and <Z_a Z_b> of the grid-neighbour pairs; plus a summary."""
     import json
-    if not os.path.exists(REF_BIN):
-        raise FileNotFoundError(REF_BIN)
-    cmd = [REF_BIN, "-c", circuit_file, "-T", f"{rep0}:{rep1}", "-g",
+    exe = REF_BIN
+    if widest_simd and os.path.exists(REF_BIN_AVX512) and host_has_avx512():
+        exe = REF_BIN_AVX512
+    if not os.path.exists(exe):
+        raise FileNotFoundError(exe)
+    cmd = [exe, "-c", circuit_file, "-T", f"{rep0}:{rep1}", "-g",
            f"{rows}:{cols}", "-p", repr(float(p_depol)), "-G",
            repr(float(gamma)), "-f", str(max_fused), "-t", str(threads)]
     out = subprocess.run(cmd, check=True, capture_output=True, text=True,

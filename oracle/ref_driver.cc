@@ -42,6 +42,17 @@
 
 namespace {
 
+// The simulator lib/simmux.h selected for this build's instruction set.
+#if defined(__AVX512F__)
+constexpr const char* kSimmuxSimulator = "SimulatorAVX512";
+#elif defined(__AVX2__)
+constexpr const char* kSimmuxSimulator = "SimulatorAVX";
+#elif defined(__SSE4_1__)
+constexpr const char* kSimmuxSimulator = "SimulatorSSE";
+#else
+constexpr const char* kSimmuxSimulator = "SimulatorBasic";
+#endif
+
 struct Options {
   std::string circuit_file;
   std::string mode = "avx";
@@ -116,6 +127,11 @@ int Run(const Options& opt) {
   bool ok = true;
 
   if (opt.passes < 0) {
+    // The whole circuit through QSimRunner::Run (its fusion included); the
+    // fused-pass count from a separate fusion of the same circuit.
+    double tf = GetTime();
+    num_fused = Fuser::FuseGates(param, circuit.num_qubits, circuit.ops).size();
+    t_fuse = GetTime() - tf;
     Sim sim = factory.CreateSimulator();
     std::vector<typename StateSpace::MeasurementResult> mres;
     t0 = GetTime();
@@ -168,11 +184,16 @@ int Run(const Options& opt) {
 
   double norm = ss.Norm(state);
   std::printf(
-      "{\"mode\": \"%s\", \"num_qubits\": %u, \"num_gates\": %zu, "
+      "{\"mode\": \"%s\", \"simulator\": \"%s\", "
+      "\"num_qubits\": %u, \"num_gates\": %zu, "
       "\"max_fused\": %u, \"threads\": %u, \"fused_ops\": %zu, "
       "\"passes\": %ld, \"fuse_s\": %.9g, \"sim_s\": %.9g, "
       "\"sample_s\": %.9g, \"norm\": %.17g}\n",
-      opt.mode.c_str(), circuit.num_qubits, circuit.ops.size(),
+      opt.mode.c_str(),
+      opt.mode == "avx" ? kSimmuxSimulator
+                        : (opt.mode == "basic64" ? "SimulatorBasic<double>"
+                                                 : "SimulatorBasic<float>"),
+      circuit.num_qubits, circuit.ops.size(),
       opt.max_fused, opt.threads, num_fused, opt.passes, t_fuse, t_sim, t_samp,
       norm);
   return 0;

@@ -84,6 +84,13 @@ SIGNATURES = [
     ("qsim_hip_real_inner_product", _i, [_i, _vp, _vp, _u, _f64p]),
     ("qsim_hip_norm", _i, [_i, _vp, _u, _f64p]),
     ("qsim_hip_sample", _i, [_i, _vp, _u, _f64p, _u64, _u64p]),
+    ("qsim_hip_set_gate_queue", _i, [_u, _u]),
+    ("qsim_hip_get_gate_queue", _i, [_vp, _vp, _vp, _vp]),
+    ("qsim_hip_flush", _i, []),
+    ("qsim_hip_block_resident_ops", _u, [_u]),
+    ("qsim_hip_costmodel_load", _i, [ctypes.c_char_p, _vp]),
+    ("qsim_hip_costmodel_set_default", _i, [_vp]),
+    ("qsim_hip_costmodel_default", _vp, []),
     ("qsim_hip_sample_seeded", _i, [_i, _vp, _u, _u64, _u, _u64p]),
     ("qsim_hip_num_partial_norms", _u, [_u]),
     ("qsim_hip_partial_norms", _i, [_i, _vp, _u, _f64p]),
@@ -177,6 +184,26 @@ def set_gate_method(method: int):
 
 def get_gate_method() -> int:
     return lib().qsim_hip_get_gate_method()
+
+
+def set_gate_queue(tile_bits: int, low_bits: int = 4):
+    """Collects apply_gate calls into blocked passes (0: off);
+    qsim_hip_set_gate_queue."""
+    _check(lib().qsim_hip_set_gate_queue(tile_bits, low_bits))
+
+
+def gate_queue_stats():
+    """(tile_bits, low_bits, blocked passes run, gates in them)."""
+    tb, lc = ctypes.c_uint(0), ctypes.c_uint(0)
+    passes, gates = ctypes.c_uint64(0), ctypes.c_uint64(0)
+    _check(lib().qsim_hip_get_gate_queue(ctypes.byref(tb), ctypes.byref(lc),
+                                         ctypes.byref(passes),
+                                         ctypes.byref(gates)))
+    return tb.value, lc.value, passes.value, gates.value
+
+
+def flush():
+    _check(lib().qsim_hip_flush())
 
 
 class PassTimer:

@@ -21,6 +21,7 @@
 #include "gate_appl.h"
 #include "io.h"
 #include "qsim_amd/block_planner.h"
+#include "qsim_amd/learned_fuser_hip.h"
 #include "qsim_amd/learned_planner.h"
 #include "qsim_amd/simulator_hip.h"
 #include "qsim_hip.h"
@@ -245,14 +246,8 @@ template <typename FP>
 int PlanBlocked(const Circuit<Operation<FP>>& c, unsigned tile_bits,
                 unsigned low_bits, std::vector<hip::BlockPass>& blocks,
                 std::vector<Step>& steps, FusedOps<FP>& single, int* trial) {
-  hip::BlockPlannerParameter bp;
-  bp.tile_bits = tile_bits;
-  bp.low_bits = low_bits;
-  // 64 greedy restarts with scores scaled in 1 +- 0.25: on the 32-qubit grid
-  // circuit 21 passes instead of 26 for 14 tile qubits with a 4-qubit low
-  // run, for about 0.12 s of planning per shape.
-  bp.restarts = 64;
-  bp.jitter = 0.5;
+  hip::BlockPlannerParameter bp =
+      hip::CandidateBlockParameter(tile_bits, low_bits);
   bp.trial = *trial;
   hip::BlockPlan plan;
   if (!hip::PlanBlocks<IO>(bp, c.num_qubits, c.ops, plan)) {
@@ -284,15 +279,6 @@ int PlanBlocked(const Circuit<Operation<FP>>& c, unsigned tile_bits,
   return QSIM_HIP_OK;
 }
 
-bool IsDiagonal(const std::vector<float>& m, unsigned k) {
-  const unsigned D = 1u << k;
-  for (unsigned r = 0; r < D; ++r)
-    for (unsigned c = 0; c < D; ++c)
-      if (r != c && (m[2 * (r * D + c)] != 0 || m[2 * (r * D + c) + 1] != 0))
-        return false;
-  return true;
-}
-
 // Cost-model features of step i when run with gate method `method`.
 void PassFeatures(const DriverCircuit& d, unsigned i, int method, float* f) {
   const unsigned n = d.c32.num_qubits;
@@ -302,35 +288,48 @@ void PassFeatures(const DriverCircuit& d, unsigned i, int method, float* f) {
     hip::BlockFeatures(n, b.tile, b.ks.size(), f);
     return;
   }
-  const auto& op = d.f32[st.index];
-  if (const auto* fg = std::get_if<FusedGate<float>>(&op)) {
-    const unsigned k = fg->qubits.size();
-    hip::GateFeatures(n, fg->qubits, 0, IsDiagonal(fg->matrix, k),
-                      hip::GateFamily(k, 0, method, n), f);
-    return;
-  }
-  const Operation<float>* o = nullptr;
-  if (const auto* pp = std::get_if<const Operation<float>*>(&op)) o = *pp;
-  if (o != nullptr) {
-    if (const auto* cg = OpGetAlternative<ControlledGate<float>>(*o)) {
-      hip::GateFeatures(n, cg->qubits, cg->controlled_by.size(), false,
-                        hip::kVector, f);
-      return;
-    }
-  }
-  // Measurements: one reduction pass over the measured qubits.
-  const std::vector<unsigned>* qs = nullptr;
-  if (const auto* m = std::get_if<Measurement>(&op)) qs = &m->qubits;
-  else if (o != nullptr) qs = &OpQubits(*o);
-  static const std::vector<unsigned> none;
-  hip::GateFeatures(n, qs ? *qs : none, 0, false, hip::kVector, f);
+  hip::FusedOpFeatures<float>(n, d.f32[st.index], method, f);
 }
 
-int ApplyCandidate(DriverCircuit& d, const hip::PlanCandidate& c) {
-  int rc = c.blocked ? qsim_driver_fuse_blocked(&d, c.a, c.b)
+int FuseBlocked(DriverCircuit& d, unsigned tile_bits, unsigned low_bits,
+                int trial);
+
+// trial >= 0: a blocked candidate's restart, as its scoring chose it.
+int ApplyCandidate(DriverCircuit& d, const hip::PlanCandidate& c,
+                   int trial = -1) {
+  int rc = c.blocked ? FuseBlocked(d, c.a, c.b, trial)
                      : qsim_driver_fuse(&d, c.a, QSIM_DRIVER_FUSER_MQUBIT,
                                         QSIM_HIP_F64);
   d.method = rc == QSIM_HIP_OK ? c.method : -1;
+  return rc;
+}
+
+int FuseBlocked(DriverCircuit& d, unsigned tile_bits, unsigned low_bits,
+                int trial) {
+  if (tile_bits < 9 || tile_bits > 14 || low_bits < 3 ||
+      low_bits > tile_bits) {
+    return QSIM_HIP_ERR_ARG;
+  }
+  d.method = -1;
+  d.matrix_prec = QSIM_HIP_F64;
+  d.ReleasePrepared();
+  std::vector<hip::BlockPass> b32;
+  std::vector<Step> s32;
+  // The FP32 copy of the circuit has the same structure: it takes the
+  // restart the FP64 search chose.
+  int rc = PlanBlocked(d.c64, tile_bits, low_bits, d.blocks, d.steps, d.f64,
+                       &trial);
+  if (rc == QSIM_HIP_OK) {
+    rc = PlanBlocked(d.c32, tile_bits, low_bits, b32, s32, d.f32, &trial);
+  }
+  if (rc == QSIM_HIP_OK && s32.size() != d.steps.size()) {
+    rc = QSIM_HIP_ERR_RUNTIME;
+  }
+  if (rc != QSIM_HIP_OK) {
+    d.f32.clear();
+    d.f64.clear();
+    PlainSteps(d);
+  }
   return rc;
 }
 
@@ -379,33 +378,7 @@ int qsim_driver_fuse(void* c, unsigned max_fused_size, int fuser,
 }
 
 int qsim_driver_fuse_blocked(void* c, unsigned tile_bits, unsigned low_bits) {
-  auto* d = static_cast<DriverCircuit*>(c);
-  if (tile_bits < 9 || tile_bits > 14 || low_bits < 3 ||
-      low_bits > tile_bits) {
-    return QSIM_HIP_ERR_ARG;
-  }
-  d->method = -1;
-  d->matrix_prec = QSIM_HIP_F64;
-  d->ReleasePrepared();
-  std::vector<hip::BlockPass> b32;
-  std::vector<Step> s32;
-  // The FP32 copy of the circuit has the same structure: it takes the
-  // restart the FP64 search chose.
-  int trial = -1;
-  int rc = PlanBlocked(d->c64, tile_bits, low_bits, d->blocks, d->steps,
-                       d->f64, &trial);
-  if (rc == QSIM_HIP_OK) {
-    rc = PlanBlocked(d->c32, tile_bits, low_bits, b32, s32, d->f32, &trial);
-  }
-  if (rc == QSIM_HIP_OK && s32.size() != d->steps.size()) {
-    rc = QSIM_HIP_ERR_RUNTIME;
-  }
-  if (rc != QSIM_HIP_OK) {
-    d->f32.clear();
-    d->f64.clear();
-    PlainSteps(*d);
-  }
-  return rc;
+  return FuseBlocked(*static_cast<DriverCircuit*>(c), tile_bits, low_bits, -1);
 }
 
 unsigned qsim_driver_num_passes(void* c) {
@@ -519,42 +492,19 @@ int qsim_driver_plan_learned(void* c, void* model, unsigned max_candidates,
                              int* chosen) {
   auto* d = static_cast<DriverCircuit*>(c);
   const unsigned n = d->c64.num_qubits;
-  std::vector<hip::PlanCandidate> cands = hip::DefaultCandidates();
-  std::vector<float> feats;
-  std::vector<unsigned> owner;
-  std::vector<int> ok(cands.size(), 0);
-  std::vector<unsigned> npass(cands.size(), 0);
-  if (model != nullptr) {
-    for (size_t j = 0; j < cands.size(); ++j) {
-      if (ApplyCandidate(*d, cands[j]) != QSIM_HIP_OK) continue;
-      ok[j] = 1;
-      npass[j] = d->steps.size();
-      for (unsigned i = 0; i < d->steps.size(); ++i) {
-        const size_t o = feats.size();
-        feats.resize(o + hip::kCostFeatures);
-        PassFeatures(*d, i, cands[j].method, &feats[o]);
-        owner.push_back(j);
-      }
-    }
-  }
-  std::vector<double> pred(cands.size(), std::nan(""));
+  const std::vector<hip::PlanCandidate> cands = hip::DefaultCandidates();
+  std::vector<hip::CandidateResult> res(cands.size());
   int best = -1;
-  if (!owner.empty()) {
-    std::vector<float> y(owner.size());
-    int rc = qsim_hip_costmodel_forward(model, feats.data(), owner.size(),
-                                        y.data());
+  if (model != nullptr) {
+    int rc = QSIM_HIP_OK;
+    best = hip::ScoreCandidates<IO>(model, n, d->c64.ops, cands, res, &rc);
     if (rc != QSIM_HIP_OK) return rc;
-    for (size_t j = 0; j < cands.size(); ++j) pred[j] = ok[j] ? 0.0 : pred[j];
-    for (size_t r = 0; r < owner.size(); ++r) {
-      pred[owner[r]] += std::exp(double(y[r]) + hip::CostTargetOffset(n));
-    }
-    for (size_t j = 0; j < cands.size(); ++j) {
-      if (ok[j] && (best < 0 || pred[j] < pred[best])) best = int(j);
-    }
   }
-  hip::PlanCandidate pick =
+  const hip::PlanCandidate pick =
       best >= 0 ? cands[best] : hip::FallbackCandidate();
-  int rc = ApplyCandidate(*d, pick);
+  const int trial =
+      best >= 0 && pick.blocked ? int(res[best].plan.trial) : -1;
+  int rc = ApplyCandidate(*d, pick, trial);
   if (rc != QSIM_HIP_OK) return rc;
   if (best < 0) {
     // No model: the fallback, reported by its index in the list.
@@ -571,8 +521,8 @@ int qsim_driver_plan_learned(void* c, void* model, unsigned max_candidates,
     if (a) a[j] = cands[j].a;
     if (b) b[j] = cands[j].b;
     if (methods) methods[j] = cands[j].method;
-    if (predicted_ms) predicted_ms[j] = pred[j];
-    if (num_passes) num_passes[j] = npass[j];
+    if (predicted_ms) predicted_ms[j] = res[j].predicted_ms;
+    if (num_passes) num_passes[j] = res[j].passes;
   }
   if (num_candidates) *num_candidates = cands.size();
   if (chosen) *chosen = best;

@@ -225,7 +225,7 @@ using namespace qhip;
 
 namespace {
 
-int CheckState(int prec, const void* s, unsigned n) {
+int CheckStateArgs(int prec, const void* s, unsigned n) {
   if (prec != QSIM_HIP_F32 && prec != QSIM_HIP_F64) {
     SetError("unknown precision %d", prec);
     return QSIM_HIP_ERR_ARG;
@@ -239,6 +239,13 @@ int CheckState(int prec, const void* s, unsigned n) {
     return QSIM_HIP_ERR_ARG;
   }
   return EnsureInit();
+}
+
+// Every entry point that touches a state (or orders work on the stream)
+// first runs the gates the queue holds.
+int CheckState(int prec, const void* s, unsigned n) {
+  QHIP_TRY(CheckStateArgs(prec, s, n));
+  return FlushGateQueue();
 }
 
 uint64_t ElemBytes(int prec) { return prec == QSIM_HIP_F32 ? 4 : 8; }
@@ -300,12 +307,14 @@ int qsim_hip_device_info(char* name, size_t name_len, uint64_t* total_mem,
 
 int qsim_hip_set_stream(void* stream) {
   QHIP_TRY(EnsureInit());
+  QHIP_TRY(FlushGateQueue());
   g_stream = static_cast<hipStream_t>(stream);
   return QSIM_HIP_OK;
 }
 
 int qsim_hip_synchronize(void) {
   QHIP_TRY(EnsureInit());
+  QHIP_TRY(FlushGateQueue());
   QHIP_CHECK(hipStreamSynchronize(g_stream));
   return QSIM_HIP_OK;
 }
@@ -337,6 +346,7 @@ int qsim_hip_alloc(int prec, unsigned num_qubits, void** out) {
 int qsim_hip_free(void* ptr) {
   if (ptr == nullptr) return QSIM_HIP_OK;
   QHIP_TRY(EnsureInit());
+  QHIP_TRY(FlushGateQueue());
   QHIP_CHECK(hipStreamSynchronize(g_stream));
   QHIP_CHECK(hipFree(ptr));
   return QSIM_HIP_OK;
@@ -344,6 +354,7 @@ int qsim_hip_free(void* ptr) {
 
 static int Copy(void* dst, const void* src, uint64_t bytes, hipMemcpyKind k) {
   QHIP_TRY(EnsureInit());
+  QHIP_TRY(FlushGateQueue());
   if (bytes == 0) return QSIM_HIP_OK;
   if (dst == nullptr || src == nullptr) {
     SetError("null copy operand");
@@ -564,12 +575,10 @@ int qsim_hip_collapse(int prec, void* s, unsigned n, uint64_t mask,
                        Collapse<double>((double2*)s, n, mask, bits));
 }
 
-int qsim_hip_apply_controlled_gate(int prec, void* s, unsigned n,
-                                   const unsigned* qs, unsigned k,
-                                   const unsigned* cqs, unsigned nc,
-                                   uint64_t cvals, const void* matrix,
-                                   int matrix_prec) {
-  QHIP_TRY(CheckState(prec, s, n));
+// One gate pass now (no queue).
+static int ApplyGateNow(int prec, void* s, unsigned n, const unsigned* qs,
+                        unsigned k, const unsigned* cqs, unsigned nc,
+                        uint64_t cvals, const void* matrix, int matrix_prec) {
   if ((k > 0 && qs == nullptr) || (nc > 0 && cqs == nullptr) ||
       matrix == nullptr) {
     SetError("null gate operand");
@@ -616,10 +625,175 @@ int qsim_hip_apply_controlled_gate(int prec, void* s, unsigned n,
   }
 }
 
+int qsim_hip_apply_controlled_gate(int prec, void* s, unsigned n,
+                                   const unsigned* qs, unsigned k,
+                                   const unsigned* cqs, unsigned nc,
+                                   uint64_t cvals, const void* matrix,
+                                   int matrix_prec) {
+  QHIP_TRY(CheckState(prec, s, n));
+  return ApplyGateNow(prec, s, n, qs, k, cqs, nc, cvals, matrix, matrix_prec);
+}
+
+}  // extern "C"
+
+namespace qhip {
+
+// ---- gate queue ------------------------------------------------------------
+//
+// While on, uncontrolled complex64 gates on 1..4 qubits are collected into
+// cache-blocked passes instead of running one pass each: consecutive gates
+// on one state whose qubits, with the low_bits lowest qubits, fit in
+// tile_bits qubits (and whose number fits the pass kernel's resident op
+// records) run as one blocked pass when the next gate does not fit, or at
+// the first other call of the library.  Grouping consecutive gates
+// greedily this way gives the fewest passes any grouping into consecutive
+// runs could (a run that fits stays fitting when shortened), so a fuser
+// that emits its gates pass by pass gets its passes back.
+struct GateQueue {
+  unsigned tb = 0, lc = 0;  // tb == 0: off
+  void* state = nullptr;
+  unsigned n = 0;
+  uint64_t mask = 0;        // qubits of the queued gates and the low run
+  std::vector<unsigned> ks, qs;
+  std::vector<double> m;
+  std::vector<size_t> moff;
+  uint64_t passes = 0, gates = 0;  // blocked passes run and gates in them
+};
+static std::mutex g_queue_mu;
+static GateQueue g_queue;
+
+static int FlushLocked() {
+  GateQueue& q = g_queue;
+  const unsigned ng = unsigned(q.ks.size());
+  if (ng == 0) return QSIM_HIP_OK;
+  int rc = QSIM_HIP_OK;
+  if (ng == 1 || q.n < kBlockMinBits) {
+    for (unsigned i = 0; i < ng && rc == QSIM_HIP_OK; ++i) {
+      rc = ::ApplyGateNow(QSIM_HIP_F32, q.state, q.n, &q.qs[4 * i], q.ks[i],
+                        nullptr, 0, 0, &q.m[q.moff[i]], QSIM_HIP_F64);
+    }
+  } else {
+    // The tile: the queued gates' qubits and the low run, filled up to
+    // tile_bits with the lowest other qubits (extending the contiguous low
+    // run where it can).
+    uint64_t mask = q.mask;
+    const unsigned want = std::min(q.tb, q.n);
+    for (unsigned b = 0; b < q.n && unsigned(__builtin_popcountll(mask)) < want;
+         ++b) {
+      mask |= uint64_t{1} << b;
+    }
+    std::vector<unsigned> tile;
+    for (unsigned b = 0; b < q.n; ++b)
+      if ((mask >> b) & 1) tile.push_back(b);
+    rc = LaunchGateBlock(static_cast<float2*>(q.state), q.n, tile.data(),
+                         unsigned(tile.size()), ng, q.ks.data(), q.qs.data(),
+                         q.m.data(), q.moff.data(), false, kBlockArithF16);
+    q.passes += 1;
+    q.gates += ng;
+  }
+  q.ks.clear();
+  q.qs.clear();
+  q.m.clear();
+  q.moff.clear();
+  q.mask = 0;
+  q.state = nullptr;
+  return rc;
+}
+
+int FlushGateQueue() {
+  std::lock_guard<std::mutex> lock(g_queue_mu);
+  return FlushLocked();
+}
+
+// Queues the gate if the queue is on and takes it; *queued says whether it
+// did.
+static int Enqueue(int prec, void* s, unsigned n, const unsigned* qs,
+                   unsigned k, const void* matrix, int matrix_prec,
+                   bool* queued) {
+  *queued = false;
+  std::lock_guard<std::mutex> lock(g_queue_mu);
+  GateQueue& q = g_queue;
+  if (q.tb == 0 || prec != QSIM_HIP_F32 || k < 1 || k > 4 ||
+      (matrix_prec != QSIM_HIP_F32 && matrix_prec != QSIM_HIP_F64) ||
+      qs == nullptr || matrix == nullptr || n < kBlockMinBits) {
+    return FlushLocked();
+  }
+  uint64_t gm = 0;
+  for (unsigned i = 0; i < k; ++i) {
+    if (qs[i] >= n || (i > 0 && qs[i] <= qs[i - 1])) return FlushLocked();
+    gm |= uint64_t{1} << qs[i];
+  }
+  const uint64_t low = (uint64_t{1} << std::min(q.lc, n)) - 1;
+  const uint64_t mask = q.mask | gm | low;
+  const bool fits = q.state == s && q.n == n &&
+                    unsigned(__builtin_popcountll(mask)) <= std::min(q.tb, n) &&
+                    q.ks.size() < BlockResidentOps(q.tb);
+  if (!fits) {
+    QHIP_TRY(FlushLocked());
+    q.state = s;
+    q.n = n;
+    q.mask = gm | low;
+  } else {
+    q.mask = mask;
+  }
+  const size_t melems = size_t{2} << (2 * k);
+  q.moff.push_back(q.m.size());
+  if (matrix_prec == QSIM_HIP_F64) {
+    const double* md = static_cast<const double*>(matrix);
+    q.m.insert(q.m.end(), md, md + melems);
+  } else {
+    const float* mf = static_cast<const float*>(matrix);
+    q.m.insert(q.m.end(), mf, mf + melems);
+  }
+  q.ks.push_back(k);
+  for (unsigned i = 0; i < 4; ++i) q.qs.push_back(i < k ? qs[i] : 0);
+  *queued = true;
+  return QSIM_HIP_OK;
+}
+
+}  // namespace qhip
+
+extern "C" {
+
+int qsim_hip_set_gate_queue(unsigned tile_bits, unsigned low_bits) {
+  if (tile_bits != 0 &&
+      (tile_bits < kBlockMinBits || tile_bits > kBlockMaxBits ||
+       low_bits < 3 || low_bits > tile_bits)) {
+    SetError("gate queue: tile_bits %u..%u (or 0 for off) and low_bits 3.."
+             "tile_bits, got %u, %u", kBlockMinBits, kBlockMaxBits, tile_bits,
+             low_bits);
+    return QSIM_HIP_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lock(g_queue_mu);
+  QHIP_TRY(FlushLocked());
+  g_queue.tb = tile_bits;
+  g_queue.lc = tile_bits == 0 ? 0 : low_bits;
+  return QSIM_HIP_OK;
+}
+
+int qsim_hip_get_gate_queue(unsigned* tile_bits, unsigned* low_bits,
+                            uint64_t* passes, uint64_t* gates) {
+  std::lock_guard<std::mutex> lock(g_queue_mu);
+  if (tile_bits) *tile_bits = g_queue.tb;
+  if (low_bits) *low_bits = g_queue.lc;
+  if (passes) *passes = g_queue.passes;
+  if (gates) *gates = g_queue.gates;
+  return QSIM_HIP_OK;
+}
+
+int qsim_hip_flush(void) { return FlushGateQueue(); }
+
+unsigned qsim_hip_block_resident_ops(unsigned tile_bits) {
+  return BlockResidentOps(tile_bits);
+}
+
 int qsim_hip_apply_gate(int prec, void* s, unsigned n, const unsigned* qs,
                         unsigned k, const void* matrix, int matrix_prec) {
-  return qsim_hip_apply_controlled_gate(prec, s, n, qs, k, nullptr, 0, 0,
-                                        matrix, matrix_prec);
+  QHIP_TRY(CheckStateArgs(prec, s, n));
+  bool queued = false;
+  QHIP_TRY(Enqueue(prec, s, n, qs, k, matrix, matrix_prec, &queued));
+  if (queued) return QSIM_HIP_OK;
+  return ApplyGateNow(prec, s, n, qs, k, nullptr, 0, 0, matrix, matrix_prec);
 }
 
 int qsim_hip_apply_gate_block(int prec, void* s, unsigned n,
@@ -756,6 +930,7 @@ int qsim_hip_get_gate_method(void) { return g_method.load(); }
 
 int qsim_hip_event_record(int slot) {
   QHIP_TRY(EnsureInit());
+  QHIP_TRY(FlushGateQueue());
   if (slot < 0 || slot >= 64) {
     SetError("event slot %d out of range", slot);
     return QSIM_HIP_ERR_ARG;
@@ -777,6 +952,7 @@ int qsim_hip_event_elapsed_ms(int a, int b, float* ms) {
 
 int qsim_hip_graph_record_begin(void** plan) {
   QHIP_TRY(EnsureInit());
+  QHIP_TRY(FlushGateQueue());
   if (plan == nullptr) {
     SetError("null plan handle");
     return QSIM_HIP_ERR_ARG;
@@ -796,6 +972,7 @@ int qsim_hip_graph_record_end(void* plan) {
     SetError("no recording in progress for this plan");
     return QSIM_HIP_ERR_ARG;
   }
+  QHIP_TRY(FlushGateQueue());
   g_plan_mode = kPlanOff;
   g_plan = nullptr;
   QHIP_CHECK(hipStreamSynchronize(g_stream));
@@ -808,6 +985,7 @@ int qsim_hip_graph_capture_begin(void* plan) {
     SetError("plan is not ready for capture");
     return QSIM_HIP_ERR_ARG;
   }
+  QHIP_TRY(FlushGateQueue());
   QHIP_CHECK(hipStreamSynchronize(g_stream));
   if (pl->stream == nullptr) QHIP_CHECK(hipStreamCreate(&pl->stream));
   pl->next = 0;
@@ -826,12 +1004,15 @@ int qsim_hip_graph_capture_end(void* plan) {
     SetError("no capture in progress for this plan");
     return QSIM_HIP_ERR_ARG;
   }
+  // Gates still queued belong to the captured sequence.
+  const int frc = FlushGateQueue();
   g_plan_mode = kPlanOff;
   g_plan = nullptr;
   g_stream = pl->saved;
   hipGraph_t graph = nullptr;
   QHIP_CHECK(hipStreamEndCapture(pl->stream, &graph));
   pl->graph = graph;
+  if (frc != QSIM_HIP_OK) return frc;
   if (pl->mismatch || pl->next != pl->bufs.size()) {
     SetError("captured passes differ from the recorded ones");
     return QSIM_HIP_ERR_ARG;
@@ -846,6 +1027,7 @@ int qsim_hip_graph_launch(void* plan) {
     SetError("plan has no captured graph");
     return QSIM_HIP_ERR_ARG;
   }
+  QHIP_TRY(FlushGateQueue());
   QHIP_CHECK(hipGraphLaunch(pl->exec, g_stream));
   return QSIM_HIP_OK;
 }
@@ -869,6 +1051,7 @@ int qsim_hip_graph_free(void* plan) {
 
 int qsim_hip_pass_timer_start(unsigned capacity) {
   QHIP_TRY(EnsureInit());
+  QHIP_TRY(FlushGateQueue());
   if (capacity == 0 || capacity > (1u << 20)) {
     SetError("pass timer capacity %u out of range", capacity);
     return QSIM_HIP_ERR_ARG;
@@ -886,6 +1069,7 @@ int qsim_hip_pass_timer_start(unsigned capacity) {
 
 int qsim_hip_pass_timer_stop(float* ms, unsigned max_count, unsigned* count) {
   QHIP_TRY(EnsureInit());
+  QHIP_TRY(FlushGateQueue());
   g_timer.on = false;
   QHIP_CHECK(hipStreamSynchronize(g_stream));
   unsigned n = std::min(g_timer.count, max_count);

@@ -656,6 +656,10 @@ void LaunchF16(float2* s, const BlockGeom& g, const BlockProgram& p,
 
 }  // namespace
 
+unsigned BlockResidentOps(unsigned tile_bits) {
+  return tile_bits == 14 ? F16Lds<14>::kMaxOps : F16Lds<13>::kMaxOps;
+}
+
 int RunGateBlock(float2* s, const BlockGeom& g, const BlockProgram& p,
                  const uint16_t* dfrag) {
   // More than 64 KiB of dynamic LDS must be allowed once per process, and

@@ -15,9 +15,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <random>
+#include <string>
 #include <vector>
 
 #include "internal.h"
@@ -306,8 +310,81 @@ int qsim_hip_costmodel_create(unsigned num_features, unsigned hidden,
   return QSIM_HIP_OK;
 }
 
+// The process's default model (qsim_hip_costmodel_set_default).
+static std::atomic<void*> g_default_model{nullptr};
+
+void* qsim_hip_costmodel_default(void) { return g_default_model.load(); }
+
+int qsim_hip_costmodel_set_default(void* model) {
+  g_default_model.store(model);
+  return QSIM_HIP_OK;
+}
+
+int qsim_hip_costmodel_load(const char* path, void** model) {
+  if (path == nullptr || model == nullptr) {
+    SetError("cost model: null path or handle");
+    return QSIM_HIP_ERR_ARG;
+  }
+  *model = nullptr;
+  std::FILE* f = std::fopen(path, "rb");
+  if (f == nullptr) {
+    SetError("cost model: cannot open %s", path);
+    return QSIM_HIP_ERR_ARG;
+  }
+  std::string text;
+  char buf[65536];
+  size_t got;
+  while ((got = std::fread(buf, 1, sizeof(buf), f)) > 0) text.append(buf, got);
+  std::fclose(f);
+  // {"num_features": F, "hidden": H, "params": [p0, p1, ...], ...}
+  auto number_after = [&](const char* key, double* v) {
+    const size_t k = text.find(key);
+    if (k == std::string::npos) return false;
+    const size_t c = text.find(':', k);
+    if (c == std::string::npos) return false;
+    char* end = nullptr;
+    *v = std::strtod(text.c_str() + c + 1, &end);
+    return end != text.c_str() + c + 1;
+  };
+  double F = 0, H = 0;
+  const size_t pk = text.find("\"params\"");
+  const size_t lb = pk == std::string::npos ? pk : text.find('[', pk);
+  if (!number_after("\"num_features\"", &F) ||
+      !number_after("\"hidden\"", &H) || lb == std::string::npos) {
+    SetError("cost model: %s lacks num_features, hidden or params", path);
+    return QSIM_HIP_ERR_ARG;
+  }
+  std::vector<float> params;
+  const char* p = text.c_str() + lb + 1;
+  while (true) {
+    while (*p == ' ' || *p == '\n' || *p == '\r' || *p == '\t' || *p == ',') ++p;
+    if (*p == ']' || *p == 0) break;
+    char* end = nullptr;
+    const double v = std::strtod(p, &end);
+    if (end == p) {
+      SetError("cost model: bad number in %s", path);
+      return QSIM_HIP_ERR_ARG;
+    }
+    params.push_back(float(v));
+    p = end;
+  }
+  void* m = nullptr;
+  QHIP_TRY(qsim_hip_costmodel_create(unsigned(F), unsigned(H), &m));
+  const int rc = qsim_hip_costmodel_set_params(m, params.data(), params.size());
+  if (rc != QSIM_HIP_OK) {
+    qsim_hip_costmodel_free(m);
+    return rc;
+  }
+  *model = m;
+  return QSIM_HIP_OK;
+}
+
 int qsim_hip_costmodel_free(void* model) {
   if (model == nullptr) return QSIM_HIP_OK;
+  {
+    void* expected = model;
+    g_default_model.compare_exchange_strong(expected, nullptr);
+  }
   auto* c = static_cast<CostModel*>(model);
   (void)hipStreamSynchronize(Stream());
   (void)hipFree(c->params);

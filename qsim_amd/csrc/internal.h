@@ -53,6 +53,13 @@ int StageMatrix(void* dev, const void* host, size_t bytes);
 
 int NumCUs();
 
+// Runs the gates the gate queue holds (qsim_hip_set_gate_queue).
+int FlushGateQueue();
+
+// Op records an FP16 blocked pass of tile_bits tile qubits keeps resident in
+// LDS (longer programs stream the state once per such chunk).
+unsigned BlockResidentOps(unsigned tile_bits);
+
 // Per-pass timer hooks around gate kernel launches (no-ops unless started).
 void PassTimerBegin();
 void PassTimerEnd();

@@ -869,6 +869,7 @@ int qsim_hip_traj_prepare(const qsim_hip_traj_desc* trajs, unsigned num_traj,
 }
 
 int qsim_hip_traj_run(void* handle, unsigned threads, int use_graph) {
+  QHIP_TRY(FlushGateQueue());
   auto* b = static_cast<TrajBatch*>(handle);
   if (b == nullptr) return QSIM_HIP_ERR_ARG;
   // Undecided slots read -1 (the decision kernel writes the reached ones).

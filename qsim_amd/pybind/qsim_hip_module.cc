@@ -10,16 +10,55 @@
 // it makes (`qsim_simulate`, `qsim_simulate_fullstate`, samplers, expectation
 // values, qtrajectory and qsimh entry points) run here unchanged.
 
+//
+// The runner's fuser is LearnedFuserHIP (include/qsim_amd/learned_fuser_hip.h):
+// for circuits of 24 qubits and more it picks the partition per circuit
+// with the cost model committed beside the package
+// (qsim_amd/data/cost_model.json, loaded as the library's default model
+// when the module is imported), including cache-blocked passes; otherwise,
+// and without the model file, it is qsim's MultiQubitGateFuser at the
+// caller's max_fused_size, as in qsim's own extension.
+
+#include <dlfcn.h>
+
+#include <string>
+
 #include "pybind_main.h"
 
 #include "fuser_mqubit.h"
 #include "gates_cirq.h"
 #include "io.h"
 #include "qsim_amd/factory_hip.h"
+#include "qsim_amd/learned_fuser_hip.h"
 #include "qtrajectory.h"
 #include "run_qsim.h"
 
-PYBIND11_MODULE(qsim_hip, m) { GPU_MODULE_BINDINGS }
+namespace {
+
+// qsim_amd/lib/qsim_hip<ext> -> qsim_amd/data/cost_model.json.
+void LoadDefaultCostModel() {
+  Dl_info info{};
+  if (dladdr(reinterpret_cast<void*>(&LoadDefaultCostModel), &info) == 0 ||
+      info.dli_fname == nullptr) {
+    return;
+  }
+  std::string path(info.dli_fname);
+  const size_t slash = path.rfind('/');
+  path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) +
+         "/../data/cost_model.json";
+  void* model = nullptr;
+  if (qsim_hip_costmodel_default() == nullptr &&
+      qsim_hip_costmodel_load(path.c_str(), &model) == QSIM_HIP_OK) {
+    qsim_hip_costmodel_set_default(model);
+  }
+}
+
+}  // namespace
+
+PYBIND11_MODULE(qsim_hip, m) {
+  GPU_MODULE_BINDINGS
+  LoadDefaultCostModel();
+}
 
 namespace qsim {
 
@@ -34,7 +73,7 @@ struct Factory {
 
   using Gate = Cirq::GateCirq<float>;
   using Operation = qsim::Operation<float>;
-  using Runner = QSimRunner<IO, MultiQubitGateFuser<IO>, Factory>;
+  using Runner = QSimRunner<IO, LearnedFuserHIP<IO>, Factory>;
   using RunnerParameter = Runner::Parameter;
   using NoisyRunner = qsim::QuantumTrajectorySimulator<IO, Runner>;
   using NoisyRunnerParameter = NoisyRunner::Parameter;

@@ -16,6 +16,13 @@
 //   hybrid      HybridSimulator::Run amplitudes (lib/hybrid.h:304-453) with the
 //               known answers of TestHybrid2
 //   sample      StateSpace::Sample on an identical state (lib/statespace.h)
+//   learned     QSimRunner with LearnedFuserHIP: blocked passes of 14 tile
+//               qubits (gates reach SimulatorHIP::ApplyGate and the gate
+//               queue groups them) on the 20-qubit grid against
+//               SimulatorBasic<double>, with and without a mid-circuit
+//               measurement
+//   learned-model  (with a cost-model file as argv[1]) the model's own pick
+//               on a 24-qubit grid against the unfused run on FactoryHIP
 //
 // Prints one JSON object per scenario and exits 0 only if all pass.
 
@@ -40,6 +47,7 @@
 #include "io.h"
 #include "parfor.h"
 #include "qsim_amd/factory_hip.h"
+#include "qsim_amd/learned_fuser_hip.h"
 #include "qtrajectory.h"
 #include "run_qsim.h"
 #include "seqfor.h"
@@ -441,12 +449,109 @@ void Scenario_Sample() {
 
 }  // namespace
 
-int main() {
+// ---- learned: QSimRunner with LearnedFuserHIP -----------------------------
+template <typename Factory, typename Fuser>
+std::pair<std::vector<std::complex<double>>, std::vector<uint64_t>> RunWith(
+    const Factory& f, const Circuit<Operation<typename Factory::fp_type>>& c,
+    const typename QSimRunner<IO, Fuser, Factory>::Parameter& p) {
+  using Runner = QSimRunner<IO, Fuser, Factory>;
+  auto ss = f.CreateStateSpace();
+  auto s = ss.Create(c.num_qubits);
+  ss.SetStateZero(s);
+  std::vector<typename Factory::StateSpace::MeasurementResult> mres;
+  bool ok = Runner::Run(p, f, c, s, mres);
+  std::vector<uint64_t> bits;
+  for (auto& m : mres) bits.push_back(ok ? m.bits : ~uint64_t{0});
+  if (!ok) bits.push_back(~uint64_t{0});
+  return {Amplitudes(f, s), bits};
+}
+
+void Scenario_Learned() {
+  bool ok = true;
+  std::ostringstream d;
+  d << "\"cases\": [";
+  for (bool mid : {false, true}) {
+    const std::string text = GridText(4, 5, 14, 0, mid);
+    auto c32 = Parse<float>(text);
+    auto c64 = Parse<double>(text);
+    using RunnerCpu = QSimRunner<IO, MultiQubitGateFuser<IO>, FactoryCPU<double>>;
+    typename RunnerCpu::Parameter pc;
+    pc.max_fused_size = 2;
+    pc.seed = 11;
+    pc.verbosity = 0;
+    auto ref = RunWith<FactoryCPU<double>, MultiQubitGateFuser<IO>>(
+        FactoryCPU<double>(), c64, pc);
+    using Learned = LearnedFuserHIP<IO>;
+    typename QSimRunner<IO, Learned, FactoryHIP<float>>::Parameter pl;
+    pl.tile_bits = 14;
+    pl.low_bits = 4;
+    pl.seed = 11;
+    pl.verbosity = 0;
+    uint64_t p0 = 0, g0 = 0, p1 = 0, g1 = 0;
+    qsim_hip_get_gate_queue(nullptr, nullptr, &p0, &g0);
+    auto got = RunWith<FactoryHIP<float>, Learned>(FactoryHIP<float>(), c32,
+                                                   pl);
+    qsim_hip_get_gate_queue(nullptr, nullptr, &p1, &g1);
+    qsim_hip_set_gate_queue(0, 0);
+    const double e = RelL2(got.first, ref.first);
+    const bool same_bits = got.second == ref.second;
+    // The fused gates came back as blocked passes (several gates each).
+    const bool blocked = p1 > p0 && g1 - g0 >= 2 * (p1 - p0);
+    ok = ok && e < 1e-5 && same_bits && blocked;
+    d << (mid ? ", " : "") << "{\"mid_measurement\": " << (mid ? "true" : "false")
+      << ", \"rel_l2_vs_fp64\": " << e << ", \"blocked_passes\": " << (p1 - p0)
+      << ", \"gates_in_blocked_passes\": " << (g1 - g0)
+      << ", \"measured_bits_equal\": " << (same_bits ? "true" : "false") << "}";
+  }
+  d << "]";
+  Report("learned", ok, d.str());
+}
+
+void Scenario_LearnedModel(const char* model_path) {
+  void* model = nullptr;
+  if (qsim_hip_costmodel_load(model_path, &model) != QSIM_HIP_OK) {
+    Report("learned-model", false,
+           std::string("\"error\": \"") + qsim_hip_last_error() + "\"");
+    return;
+  }
+  qsim_hip_costmodel_set_default(model);
+  const std::string text = GridText(4, 6, 12, 5, false);
+  auto c = Parse<float>(text);
+  // The unfused run: every gate of the circuit on its own.
+  FactoryHIP<float> f;
+  auto ss = f.CreateStateSpace();
+  auto sim = f.CreateSimulator();
+  auto s = ss.Create(c.num_qubits);
+  ss.SetStateZero(s);
+  for (const auto& op : c.ops) ApplyGate(sim, op, s);
+  auto ref = Amplitudes(f, s);
+  using Learned = LearnedFuserHIP<IO>;
+  typename QSimRunner<IO, Learned, FactoryHIP<float>>::Parameter pl;
+  pl.seed = 1;
+  pl.verbosity = 0;  // the library's default model
+  unsigned tb = 0, lb = 0;
+  uint64_t p0 = 0, g0 = 0, p1 = 0, g1 = 0;
+  qsim_hip_get_gate_queue(nullptr, nullptr, &p0, &g0);
+  auto got = RunWith<FactoryHIP<float>, Learned>(f, c, pl);
+  qsim_hip_get_gate_queue(&tb, &lb, &p1, &g1);
+  qsim_hip_set_gate_queue(0, 0);
+  qsim_hip_costmodel_free(model);
+  const double e = RelL2(got.first, ref);
+  std::ostringstream d;
+  d << "\"rel_l2_vs_unfused\": " << e << ", \"tile_bits\": " << tb
+    << ", \"low_bits\": " << lb << ", \"blocked_passes\": " << (p1 - p0)
+    << ", \"gates_in_blocked_passes\": " << (g1 - g0);
+  Report("learned-model", e < 1e-5, d.str());
+}
+
+int main(int argc, char** argv) {
   Scenario_Runner();
   Scenario_Callback();
   Scenario_Expect();
   Scenario_Trajectories();
   Scenario_Hybrid();
   Scenario_Sample();
+  Scenario_Learned();
+  if (argc > 1) Scenario_LearnedModel(argv[1]);
   return g_failures == 0 ? 0 : 1;
 }

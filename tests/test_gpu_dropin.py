@@ -1,7 +1,7 @@
 """qsim's own drivers (QSimRunner, ExpectationValue, QuantumTrajectorySimulator,
 HybridSimulator, StateSpace::Sample/Measure) running on FactoryHIP, compared
 in one binary against qsim's CPU SimulatorBasic Factory and qsim's known
-answers (qsim_amd/apps/dropin_check.cc)."""
+answers (tests/dropin/dropin_check.cc)."""
 
 import json
 import os
@@ -22,10 +22,12 @@ BIN = os.path.join(ROOT, "qsim_amd", "bin")
 
 @pytest.fixture(scope="module")
 def dropin_results():
-    exe = os.path.join(BIN, "qsim_dropin_check")
+    exe = os.path.join(ROOT, "tests", "dropin", "bin", "qsim_dropin_check")
     if not os.path.exists(exe):
         pytest.skip("drop-in binary not built (needs the qsim headers)")
-    r = subprocess.run([exe], capture_output=True, text=True, timeout=240)
+    model = os.path.join(ROOT, "qsim_amd", "data", "cost_model.json")
+    r = subprocess.run([exe, model], capture_output=True, text=True,
+                       timeout=240)
     print(r.stdout, r.stderr, flush=True)
     res = {}
     for line in r.stdout.splitlines():
@@ -35,8 +37,14 @@ def dropin_results():
     return r.returncode, res
 
 
+# learned: QSimRunner<IO, LearnedFuserHIP<IO>, FactoryHIP<float>> reaches
+# blocked passes through SimulatorHIP::ApplyGate (the library's gate queue)
+# and matches SimulatorBasic<double> within 1e-5 relative L2 on the 20-qubit
+# grid, measurements included; learned-model: the committed cost model's
+# own pick at 24 qubits within 1e-5 of the unfused run.
 @pytest.mark.parametrize("scenario", ["runner", "callback", "expect",
-                                      "qtrajectory", "hybrid", "sample"])
+                                      "qtrajectory", "hybrid", "sample",
+                                      "learned", "learned-model"])
 def test_qsim_driver_on_hip_factory(dropin_results, scenario):
     rc, res = dropin_results
     assert scenario in res, f"{scenario} did not report"
