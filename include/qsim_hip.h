@@ -43,8 +43,11 @@ enum {
   QSIM_HIP_METHOD_MFMA_BF16X3 = 2,  /* as AUTO, but k >= 5 on split-BF16
                                        MFMA (matrix in 3 bf16 parts, state in
                                        2, 4 products) with FP32 accumulation */
-  QSIM_HIP_METHOD_MFMA_FP16X2 = 3   /* reserved: scaled two-term FP16 MFMA
-                                       (selecting it is rejected) */
+  QSIM_HIP_METHOD_MFMA_FP16X2 = 3   /* as AUTO, but k >= 5 on FP16 MFMA
+                                       with a power-of-two scale per matrix
+                                       and per group (matrix and state in 2
+                                       fp16 parts each, 3 products) and FP32
+                                       accumulation */
 };
 
 /* ---- context ----------------------------------------------------------- */

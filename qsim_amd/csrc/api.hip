@@ -630,6 +630,10 @@ int qsim_hip_apply_controlled_gate(int prec, void* s, unsigned n,
                                    const unsigned* cqs, unsigned nc,
                                    uint64_t cvals, const void* matrix,
                                    int matrix_prec) {
+  // Without controls this is a plain gate, which the gate queue may take.
+  if (nc == 0) {
+    return qsim_hip_apply_gate(prec, s, n, qs, k, matrix, matrix_prec);
+  }
   QHIP_TRY(CheckState(prec, s, n));
   return ApplyGateNow(prec, s, n, qs, k, cqs, nc, cvals, matrix, matrix_prec);
 }
@@ -917,10 +921,6 @@ int qsim_hip_set_gate_method(int method) {
   if (method < QSIM_HIP_METHOD_AUTO || method > QSIM_HIP_METHOD_MFMA_FP16X2) {
     SetError("unknown gate method %d", method);
     return QSIM_HIP_ERR_ARG;
-  }
-  if (method == QSIM_HIP_METHOD_MFMA_FP16X2) {
-    SetError("the scaled FP16 MFMA method is not built");
-    return QSIM_HIP_ERR_UNSUPPORTED;
   }
   g_method.store(method);
   return QSIM_HIP_OK;

@@ -203,6 +203,12 @@ int Convert(void* dst, int dprec, const void* src, int sprec, unsigned n);
 int LaunchGateMfma(float2* s, unsigned n, const unsigned* qs, unsigned k,
                    const double* m64, int method);
 
+// FP16 MFMA with per-group scaling for uncontrolled k = 5, 6 gates on
+// complex64 states (f16_kernels.hip); QSIM_HIP_ERR_UNSUPPORTED when the
+// state is too small for its batches.
+int LaunchGateF16(float2* s, unsigned n, const unsigned* qs, unsigned k,
+                  const double* m64);
+
 // Split-BF16 A-operand fragments of a 2^K x 2^K complex matrix (FP64,
 // row-major interleaved): 2^K/16 row blocks x 2^(K+1)/16 K blocks x
 // (hi, mid, lo) x 512.

@@ -522,6 +522,10 @@ int LaunchGateMfma(float2* s, unsigned n, const unsigned* qs, unsigned k,
       return QSIM_HIP_ERR_ARG;
     }
   }
+  if (method == QSIM_HIP_METHOD_MFMA_FP16X2 && k >= 5) {
+    const int rc = LaunchGateF16(s, n, qs, k, m64);
+    if (rc != QSIM_HIP_ERR_UNSUPPORTED) return rc;
+  }
   const bool bf16 = method == QSIM_HIP_METHOD_MFMA_BF16X3 && k >= 5;
   switch (k) {
     case 4: return DispatchMfma<4>(s, n, qs, m64, false);

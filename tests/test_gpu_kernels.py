@@ -442,8 +442,49 @@ def test_bf16x3_mfma_accuracy(sim):
         qsim_amd.set_gate_method(qsim_amd.METHOD_AUTO)
 
 
-def test_fp16_method_is_rejected():
-    with pytest.raises(qsim_amd.QsimHipError) as e:
-        qsim_amd.set_gate_method(qsim_amd.METHOD_MFMA_FP16X2)
-    assert e.value.code == -4
-    assert qsim_amd.get_gate_method() == qsim_amd.METHOD_AUTO
+def test_fp16x2_mfma_accuracy(sim):
+    # FP16 with a power-of-two scale per matrix and per group, matrix and
+    # amplitudes in two fp16 parts each (22 bits), FP32 accumulation: per
+    # pass the state stays within 1e-6 relative L2 of the FP64 result, for
+    # low, high, mixed and random targets (LDS-tile and register paths).
+    rng = np.random.default_rng(23)
+    qsim_amd.set_gate_method(qsim_amd.METHOD_MFMA_FP16X2)
+    try:
+        for n in (11, 12, 13, 16, 20):
+            for k in (5, 6):
+                for qs in _target_sets(n, k, rng):
+                    u = random_matrix(k, rng)
+                    s = random_state(n, rng, np.float64)
+                    st = upload(s.astype(np.float32), n, np.float32)
+                    sim.apply_gate(qs, to_interleaved(u, np.float64), st)
+                    oracle.apply_gate(n, s, qs, to_interleaved(u, np.float64))
+                    err = rel_l2(st.to_numpy(), s)
+                    assert err < 1e-6, (n, k, qs, err)
+    finally:
+        qsim_amd.set_gate_method(qsim_amd.METHOD_AUTO)
+
+
+@pytest.mark.parametrize("scale", [1e-30, 1e-3, 1.0, 1e3, 1e30])
+def test_fp16x2_scaling_over_magnitudes(sim, scale):
+    # The per-group scale keeps fp16's 5-bit exponent out of the way: groups
+    # of very different size in one state (one group all zero, one tiny, the
+    # rest at `scale`), and non-unitary matrices scaled by 1e-3 and 1e3.
+    rng = np.random.default_rng(24)
+    n = 14
+    qsim_amd.set_gate_method(qsim_amd.METHOD_MFMA_FP16X2)
+    try:
+        for k, qs in ((5, [0, 3, 6, 9, 12]), (6, [5, 6, 7, 8, 9, 13]),
+                      (6, [0, 1, 2, 3, 4, 5])):
+            for mscale in (1e-3, 1.0, 1e3):
+                u = random_matrix(k, rng) * mscale
+                s = random_state(n, rng, np.float64) * scale
+                s[: 2 << k] = 0.0           # a zero group (low targets)
+                s[2 << k: 4 << k] *= 1e-20  # a tiny one
+                s = s.astype(np.float32).astype(np.float64)
+                st = upload(s.astype(np.float32), n, np.float32)
+                sim.apply_gate(qs, to_interleaved(u, np.float64), st)
+                oracle.apply_gate(n, s, qs, to_interleaved(u, np.float64))
+                err = rel_l2(st.to_numpy(), s)
+                assert err < 1e-6, (k, qs, mscale, err)
+    finally:
+        qsim_amd.set_gate_method(qsim_amd.METHOD_AUTO)
