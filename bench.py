@@ -10,6 +10,9 @@ apply loop over the gfx950 kernels.  The paths run on the same circuit:
   fp32-mfma          max_fused_size 4, exact-FP32 MFMA for the k=4 passes
   bf16-k6            max_fused_size 6, fused matrices built in FP64 on the
                      host, split-BF16 MFMA for k>=5 passes
+  fp16-k5, fp16-k6   max_fused_size 5 / 6, FP64-built fused matrices, FP16
+                     MFMA with a power-of-two scale per matrix and per group
+                     for k>=5 passes
   blocked-13         cache-blocked passes over 13 tile qubits (5 lowest + 8
                      chosen), gates fused to <= 4 qubits in FP64 inside each
                      pass, split-BF16 MFMA on the LDS-resident tile
@@ -63,6 +66,8 @@ PATHS = {
     "plain-fp32-vector": ("mqubit", 4, False, 1, "f16"),
     "fp32-mfma": ("mqubit", 4, False, 0, "f16"),
     "bf16-k6": ("mqubit", 6, True, 2, "f16"),
+    "fp16-k5": ("mqubit", 5, True, 3, "f16"),
+    "fp16-k6": ("mqubit", 6, True, 3, "f16"),
     "blocked-13": ("blocked", 13, 5, 0, "f16"),
     "blocked-14": ("blocked", 14, 4, 0, "f16"),
     "blocked-14-bf16": ("blocked", 14, 4, 0, "bf16"),
@@ -122,6 +127,8 @@ def kernel_family(circ, i, qubits, method, n):
     if method != 1 and 4 <= k <= 6 and n >= k + 5:
         if method == 2 and k >= 5:
             return "GateBf16TileKernel" if low else "GateBf16RegKernel"
+        if method == 3 and k >= 5 and n >= 12:
+            return "GateF16SetKernel" if low else "GateF16RegKernel"
         return "GateMfmaTileKernel" if low else "GateMfmaRegKernel"
     return "GateTileKernel" if low else "GateRegKernel"
 
@@ -216,6 +223,8 @@ def run_path(args, text, name):
         "num_qubits": n, "num_gates": circ.num_gates, "passes": npass,
         "circuit_s": total_ms / args.steps / 1e3, "wall_s": wall,
         "final_norm": norm, "fuse_s": fuse_s,
+        # host planning (fusion or the learned pick) plus one circuit
+        "plan_plus_circuit_s": fuse_s + total_ms / args.steps / 1e3,
         "pass_ms_sum": float(np.sum(pass_ms)) / args.steps,
         "hbm_gbps_avg_pass": bytes_per_pass / (float(np.mean(pass_ms)) * 1e-3) / 1e9,
         "gate_passes_per_s": npass / (total_ms / args.steps / 1e3),
@@ -232,11 +241,26 @@ def run_path(args, text, name):
     }
 
 
+def _rel_diff64(ss64, a, b):
+    """||a - b|| / ||b|| of two complex128 states on the device; a is
+    overwritten with a - b (b is negated and restored, exactly)."""
+    nb = ss64.norm(b)
+    ss64.multiply(-1.0, b)
+    ss64.add(b, a)
+    ss64.multiply(-1.0, b)
+    return float(np.sqrt(ss64.norm(a) / nb))
+
+
 def fidelities(args, names, picks=None):
     """1 - F (normalised states) and the norm drift of each path at
-    --fidelity-qubits qubits.  A path with a planner pick in `picks` (the
-    learned planner's choice at the benchmark size) is measured with that
-    same partition kind, not re-planned at the smaller size."""
+    --fidelity-qubits qubits against the complex128 run (FP64 fused
+    matrices, FP64 kernels; the tests pin that run to qsim's
+    SimulatorBasic<double> at 28 qubits within 1e-12), and each path's
+    relative L2 distance from the unfused complex64 run of the same circuit
+    (every gate its own FP32 pass, as QSimRunner would apply the circuit
+    without fusion).  A path with a planner pick in `picks` (the learned
+    planner's choice at the benchmark size) is measured with that same
+    partition kind, not re-planned at the smaller size."""
     import qsim_amd
     from qsim_amd import circuits, driver
 
@@ -251,6 +275,17 @@ def fidelities(args, names, picks=None):
     out = {}
     ss32 = qsim_amd.StateSpace(np.float32)
     tmp = ss64.create(nq)
+    unfused = ss64.create(nq)
+    cu = driver.Circuit(text)
+    cu.fuse(1, driver.FUSER_NONE)
+    st = ss32.create(nq)
+    try:
+        ss32.set_state_zero(st)
+        cu.run(st)
+        ss64.copy(st, unfused)
+    finally:
+        st.free()
+    log(f"unfused complex64 run at {nq}q: {cu.num_passes} passes")
     for name in names:
         c = driver.Circuit(text)
         pick = (picks or {}).get(name)
@@ -273,13 +308,17 @@ def fidelities(args, names, picks=None):
         n32 = ss64.norm(tmp)
         f_raw = abs(ip) ** 2
         f = f_raw / (n64 * n32)
+        rel_unfused = _rel_diff64(ss64, tmp, unfused)
         out[name] = {"qubits": nq, "one_minus_F": 1.0 - f,
                      "norm_drift": abs(n32 - 1.0),
-                     "one_minus_F_unnormalised": 1.0 - f_raw, "norm": n32}
+                     "one_minus_F_unnormalised": 1.0 - f_raw, "norm": n32,
+                     "rel_l2_vs_unfused": rel_unfused}
         log(f"[{name}] {nq}q: 1-F = {1.0 - f:.3e}, norm drift "
-            f"{abs(n32 - 1.0):.3e} (1-|<psi64|psi>|^2 = {1.0 - f_raw:.3e})")
+            f"{abs(n32 - 1.0):.3e} (1-|<psi64|psi>|^2 = {1.0 - f_raw:.3e}), "
+            f"rel-L2 vs unfused {rel_unfused:.2e}")
     ref.free()
     tmp.free()
+    unfused.free()
     return out
 
 

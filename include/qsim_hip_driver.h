@@ -18,7 +18,11 @@ extern "C" {
 /* Fuser selection for qsim_driver_fuse. */
 enum {
   QSIM_DRIVER_FUSER_MQUBIT = 0,  /* qsim's MultiQubitGateFuser */
-  QSIM_DRIVER_FUSER_LEARNED = 1  /* cost-model planner (qsim_driver_plan_learned) */
+  QSIM_DRIVER_FUSER_LEARNED = 1, /* cost-model planner (qsim_driver_plan_learned) */
+  QSIM_DRIVER_FUSER_NONE = 2     /* no fusion: every operation of the circuit
+                                    is its own pass, as a QSimRunner loop over
+                                    the unfused operations would apply them
+                                    (max_fused_size is ignored) */
 };
 
 /* Parses qsim's circuit text format up to `maxtime` (CircuitQsimParser::

@@ -93,6 +93,12 @@ int Fuse(const Circuit<Operation<FP>>& c, unsigned max_fused, int fuser,
     p.max_fused_size = max_fused;
     p.verbosity = 0;
     out = MultiQubitGateFuser<IO>::FuseGates(p, c.num_qubits, c.ops);
+  } else if (fuser == QSIM_DRIVER_FUSER_NONE) {
+    // Each operation as qsim's fuser hands back an unfused one: a pointer
+    // into the circuit, applied by ApplyGate (lib/gate_appl.h:30-46).
+    out.clear();
+    out.reserve(c.ops.size());
+    for (const auto& op : c.ops) out.push_back(&op);
   } else {
     return QSIM_HIP_ERR_ARG;
   }
@@ -461,7 +467,7 @@ int qsim_driver_run_passes(void* c, void* state, int prec, unsigned first,
 
 int qsim_driver_set_method(void* c, int method) {
   auto* d = static_cast<DriverCircuit*>(c);
-  if (method < -1 || method > QSIM_HIP_METHOD_MFMA_BF16X3) {
+  if (method < -1 || method > QSIM_HIP_METHOD_MFMA_FP16X2) {
     return QSIM_HIP_ERR_ARG;
   }
   d->method = method;

@@ -30,10 +30,10 @@
 //    256 bytes or more; they are copied to LDS (member-major), the next set
 //    is fetched into registers while this one is computed, and results go
 //    straight to HBM (the whole set is in LDS before any of it is written);
-//  - a target below qubit 5 (GateF16TileKernel): a tile of 2^12 or 2^13
-//    amplitudes (the Lc lowest qubits times the h targets above them, as in
-//    the FP32 tile kernels) is staged in LDS, its batches are computed in
-//    place there, and the tile is written back.
+//  - a target below qubit 5 (GateF16SetKernel): the same sets of batches,
+//    which then span 2^11 amplitudes over the targets and the lowest other
+//    qubits, are copied to LDS, computed in place there and copied back
+//    (coalesced both ways), with the next set's loads in flight meanwhile.
 
 #include <hip/hip_runtime.h>
 
@@ -43,7 +43,6 @@
 #include <vector>
 
 #include "internal.h"
-#include "tile.h"
 
 namespace qhip {
 
@@ -79,55 +78,67 @@ struct F16Frag {
   }
 };
 
-// Largest |re| or |im| of this lane's and its partner lane's (l ^ 32)
-// members: the group's; then the group's scale exponent (largest value to
-// [2^5, 2^6); 0 for an all-zero or non-finite group).
-template <int N>
-__device__ __forceinline__ int GroupScale(const float2 (&v)[N]) {
+// A lane reads its batch's inputs through load(t), t < MPL: member
+// 8 (t / 4) + 4 (l / 32) + t % 4 of its group l % 32 (lane half h supplies
+// members 8 kb + 4 h + q of K block kb).  The inputs are read twice, for
+// the group's scale and then K block by K block into the products, so that
+// only one K block's values are live at a time.
+
+// The group's scale exponent: its largest |re| or |im| (this lane's and its
+// partner lane's l ^ 32 members) to [2^5, 2^6); 0 for an all-zero or
+// non-finite group.
+template <int K, typename Load>
+__device__ __forceinline__ int GroupScale(Load load) {
   float m = 0.0f;
 #pragma unroll
-  for (int i = 0; i < N; ++i) m = fmaxf(m, fmaxf(fabsf(v[i].x), fabsf(v[i].y)));
+  for (int t = 0; t < F16Dims<K>::MPL; ++t) {
+    const float2 v = load(t);
+    m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+  }
   m = fmaxf(m, __shfl_xor(m, 32));
+  // The products read the inputs again (keeping them from here on would
+  // hold every K block's values in registers at once).
+  asm volatile("" ::: "memory");
   if (!(m > 0.0f) || !(m <= 3.4e38f)) return 0;
   int e;
   (void)frexpf(m, &e);  // m in [2^(e-1), 2^e)
   return 6 - e;
 }
 
-// The scaled inputs of K block kb as fp16 (high, low) B operands: lane half
-// h supplies members 8 kb + 4 h + q, q = 0..3, as (re, im) pairs.
-template <int N>
-__device__ __forceinline__ void SplitB(const float2 (&v)[N], int kb, int e,
-                                       v8h& bh, v8h& bl) {
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const float re = ldexpf(v[4 * kb + q].x, e);
-    const float im = ldexpf(v[4 * kb + q].y, e);
-    const _Float16 hr = (_Float16)re, hi = (_Float16)im;
-    bh[2 * q] = hr;
-    bh[2 * q + 1] = hi;
-    bl[2 * q] = (_Float16)(re - (float)hr);
-    bl[2 * q + 1] = (_Float16)(im - (float)hi);
-  }
-}
-
-// This wave's 32 output rows of one batch: rows 32 rb + r; lane l receives
-// the (re, im) pairs of members 16 rb + 4 s + 2 (l/32) + u (s < 4, u < 2)
-// of its group l % 32, handed to store(member, value) scaled by 2^-us.
-template <int K, typename Store>
-__device__ __forceinline__ void RowBlock(const F16Frag<K>& a,
-                                         const float2 (&v)[F16Dims<K>::MPL],
-                                         int rb, int e, int us, Store store) {
-  const int h = (threadIdx.x >> 5) & 1;
+// This wave's 32 output rows (32 rb .. 32 rb + 31) of the batch: K block
+// kb's inputs scaled by 2^e and split into fp16 (high, low) B operands,
+// then Al Bh + Ah Bl + Ah Bh into the FP32 accumulators.
+template <int K, typename Load>
+__device__ __forceinline__ v16f RowBlock(const F16Frag<K>& a, Load load,
+                                         int e) {
   v16f c = v16f{};
 #pragma unroll
   for (int kb = 0; kb < F16Dims<K>::KB; ++kb) {
     v8h bh, bl;
-    SplitB(v, kb, e, bh, bl);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float2 v = load(4 * kb + q);
+      const float re = ldexpf(v.x, e), im = ldexpf(v.y, e);
+      const _Float16 hr = (_Float16)re, hi = (_Float16)im;
+      bh[2 * q] = hr;
+      bh[2 * q + 1] = hi;
+      bl[2 * q] = (_Float16)(re - (float)hr);
+      bl[2 * q + 1] = (_Float16)(im - (float)hi);
+    }
     c = __builtin_amdgcn_mfma_f32_32x32x16_f16(a.f[kb][1], bh, c, 0, 0, 0);
     c = __builtin_amdgcn_mfma_f32_32x32x16_f16(a.f[kb][0], bl, c, 0, 0, 0);
     c = __builtin_amdgcn_mfma_f32_32x32x16_f16(a.f[kb][0], bh, c, 0, 0, 0);
   }
+  return c;
+}
+
+// Lane l's results: the (re, im) pairs of members 16 rb + 4 s + 2 (l/32)
+// + u (s < 4, u < 2) of its group, handed to store(member, value) scaled
+// by 2^-us.
+template <typename Store>
+__device__ __forceinline__ void StoreRows(const v16f& c, int rb, int us,
+                                          Store store) {
+  const int h = (threadIdx.x >> 5) & 1;
 #pragma unroll
   for (int s = 0; s < 4; ++s)
 #pragma unroll
@@ -187,17 +198,17 @@ GateF16RegKernel(float2* __restrict__ s, GateGeom g,
   for (; set < nsets; set += gridDim.x) {
     const uint64_t nxt = set + gridDim.x;
     if (nxt < nsets) fetch(nxt, pre);
-    // This wave's batch: groups bsub * 32 + j of the set.
-    const float2* b = buf[cur];
+    // This wave's batch: groups bsub * 32 + j of the set, read once.
+    const float2* b = buf[cur] + bsub * 32 + j;
     float2 v[Dm::MPL];
 #pragma unroll
-    for (int kb = 0; kb < Dm::KB; ++kb)
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-        v[4 * kb + q] = b[(8 * kb + 4 * h + q) * G + bsub * 32 + j];
-    const int e = GroupScale(v);
+    for (int t = 0; t < Dm::MPL; ++t)
+      v[t] = b[(8 * (t >> 2) + 4 * h + (t & 3)) * G];
+    auto load = [&](int t) { return v[t]; };
+    const int e = GroupScale<K>(load);
+    const v16f c = RowBlock<K>(a, load, e);
     const uint64_t base = InsertZeros(set * G + bsub * 32 + j, g.ins, g.nins);
-    RowBlock<K>(a, v, rb, e, e + sm, [&](int m, float2 x) {
+    StoreRows(c, rb, e + sm, [&](int m, float2 x) {
       s[base + MemberOffset(m, g.tbit, K)] = x;
     });
     if (nxt < nsets) put(pre, buf[cur ^ 1]);
@@ -206,56 +217,99 @@ GateF16RegKernel(float2* __restrict__ s, GateGeom g,
   }
 }
 
-// A target below qubit 5: LDS tiles of 2^tb amplitudes (tb = 12 or 13).
+// A target below qubit 5.  Any 32 consecutive groups of a k-qubit gate
+// span the 2^(k+5) amplitudes over the targets and the 5 lowest other
+// qubits; a workgroup's 4 / RB batches span the 2^11 amplitudes over the
+// targets and the 11 - k lowest other qubits (the set's qubits, which
+// always include qubits 0..4, so a set is runs of 32 or more contiguous
+// amplitudes).  A set is copied to LDS member-major (slot m G + g, the
+// group index XOR-ed with m's low bits so that both the batch reads and
+// the copies spread over the banks), its batches are computed, the results
+// go back to their slots and the set is copied out; the next set's loads
+// are in flight in registers meanwhile.
+struct SetGeom {
+  uint64_t nsets;
+  unsigned pos[11];  // the set's qubits, ascending
+  unsigned tmask;    // bit b: pos[b] is a target (member bits in order)
+};
+
 template <int K>
 __global__ void __launch_bounds__(256)
-GateF16TileKernel(float2* __restrict__ s, TileGeom g,
-                  const uint4* __restrict__ frag, int sm) {
+GateF16SetKernel(float2* __restrict__ s, SetGeom g,
+                 const uint4* __restrict__ frag, int sm) {
   using Dm = F16Dims<K>;
-  extern __shared__ char lds_raw[];
-  float2* lds = reinterpret_cast<float2*>(lds_raw);
+  constexpr int G = Dm::G;
+  constexpr int PERT = 2048 / 256;  // amplitudes each thread copies
+  __shared__ float2 buf[2][2048];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int rb = wave % Dm::RB, bsub = wave / Dm::RB;
   const int j = lane & 31, h = lane >> 5;
   F16Frag<K> a;
   a.Load(frag, rb);
-  const unsigned nbatch = ((1u << g.tb) >> K) >> 5;
-  constexpr int PER = 8192 / 256;  // tiles of up to 2^13 amplitudes
-  auto toff = [&](int m) {
-    unsigned o = 0;
-#pragma unroll
-    for (int b = 0; b < K; ++b) o |= ((unsigned(m) >> b) & 1u) << g.tpos[b];
-    return o;
+  auto swz = [](unsigned m, unsigned grp) {
+    return m * unsigned(G) + (grp ^ (m & 31u));
   };
-  for (uint64_t tile = blockIdx.x; tile < g.ntiles; tile += gridDim.x) {
-    const uint64_t base =
-        InsertZeros(tile << g.Lc, g.outer, g.nouter) | g.cbits_high;
-    TileLoad<PER>(s, base, g, lds);
-    __syncthreads();
-    for (unsigned bt0 = 0; bt0 < nbatch; bt0 += Dm::BPI) {
-      unsigned p0 = (bt0 + bsub) * 32 + j;
+  // Copy map: element r of this thread is set-local amplitude
+  // x = t + 256 r; its offset in the set and its LDS slot.
+  uint64_t goff[PERT];
+  unsigned slot[PERT];
 #pragma unroll
-      for (int b = 0; b < K; ++b) {
-        const unsigned p = g.tpos[b];
-        p0 = ((p0 >> p) << (p + 1)) | (p0 & ((1u << p) - 1));
+  for (int r = 0; r < PERT; ++r) {
+    const unsigned x = threadIdx.x + 256u * r;
+    uint64_t o = 0;
+    unsigned m = 0, grp = 0, mi = 0, gi = 0;
+    for (int b = 0; b < 11; ++b) {
+      const unsigned bit = (x >> b) & 1u;
+      o |= uint64_t(bit) << g.pos[b];
+      if ((g.tmask >> b) & 1u) {
+        m |= bit << mi++;
+      } else {
+        grp |= bit << gi++;
       }
-      float2 v[Dm::MPL];
-#pragma unroll
-      for (int kb = 0; kb < Dm::KB; ++kb)
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-          v[4 * kb + q] = lds[TileSwz(p0 + toff(8 * kb + 4 * h + q))];
-      // Every wave has read this step's batches before any is overwritten.
-      __syncthreads();
-      const int e = GroupScale(v);
-      RowBlock<K>(a, v, rb, e, e + sm, [&](int m, float2 x) {
-        lds[TileSwz(p0 + toff(m))] = x;
-      });
     }
+    goff[r] = o;
+    slot[r] = swz(m, grp);
+  }
+  float2 pre[PERT];
+  auto fetch = [&](uint64_t set) {
+    const float2* p = s + InsertZeros(set, g.pos, 11);
+#pragma unroll
+    for (int r = 0; r < PERT; ++r) pre[r] = p[goff[r]];
+  };
+  auto put = [&](float2* b) {
+#pragma unroll
+    for (int r = 0; r < PERT; ++r) b[slot[r]] = pre[r];
+  };
+  uint64_t set = blockIdx.x;
+  if (set < g.nsets) {
+    fetch(set);
+    put(buf[0]);
+  }
+  __syncthreads();
+  int cur = 0;
+  const unsigned grp = unsigned(bsub) * 32u + unsigned(j);
+  for (; set < g.nsets; set += gridDim.x) {
+    const uint64_t nxt = set + gridDim.x;
+    if (nxt < g.nsets) fetch(nxt);
+    float2* b = buf[cur];
+    float2 v[Dm::MPL];
+#pragma unroll
+    for (int t = 0; t < Dm::MPL; ++t)
+      v[t] = b[swz(8u * (t >> 2) + 4u * h + (t & 3), grp)];
+    auto load = [&](int t) { return v[t]; };
+    const int e = GroupScale<K>(load);
+    const v16f c = RowBlock<K>(a, load, e);
+    // Every wave has read the set before any result overwrites it.
     __syncthreads();
-    TileStore<PER>(s, base, g, lds);
+    StoreRows(c, rb, e + sm, [&](int m, float2 x) { b[swz(m, grp)] = x; });
     __syncthreads();
+    float2* p = s + InsertZeros(set, g.pos, 11);
+#pragma unroll
+    for (int r = 0; r < PERT; ++r) p[goff[r]] = b[slot[r]];
+    if (nxt < g.nsets) put(buf[cur ^ 1]);
+    __syncthreads();
+    cur ^= 1;
   }
 }
 
@@ -328,67 +382,30 @@ int DispatchF16(float2* s, unsigned n, const unsigned* qs, const double* m64) {
     return QSIM_HIP_OK;
   }
 
-  // Tile of 2^12 amplitudes, or 2^13 where that leaves fewer memory runs
-  // (and always for k = 6): 2^h runs of 2^Lc, h = targets at or above Lc.
-  auto split = [&](unsigned tb, unsigned* hout, unsigned* lcout) {
-    for (unsigned hh = 0; hh <= unsigned(K); ++hh) {
-      const unsigned lc = tb - hh;
-      unsigned cnt = 0;
-      for (int b = 0; b < K; ++b) cnt += qs[b] >= lc;
-      if (cnt == hh) {
-        *hout = hh;
-        *lcout = lc;
-        return;
-      }
-    }
-    *hout = 0;
-    *lcout = tb;
-  };
-  if (n < 12) return QSIM_HIP_ERR_UNSUPPORTED;
-  unsigned TB = 12, h = 0, Lc = 12;
-  split(TB, &h, &Lc);
-  if (n >= 13) {
-    unsigned h13, lc13;
-    split(13, &h13, &lc13);
-    if (K >= 6 || h13 < h) {
-      TB = 13;
-      h = h13;
-      Lc = lc13;
+  // The set's qubits: the targets and the 11 - k lowest other qubits.
+  if (n < 11) return QSIM_HIP_ERR_UNSUPPORTED;
+  SetGeom g{};
+  g.nsets = uint64_t{1} << (n - 11);
+  unsigned np = 0, others = 0, ti = 0;
+  for (unsigned q = 0; q < n && np < 11; ++q) {
+    const bool target = ti < unsigned(K) && qs[ti] == q;
+    if (target) {
+      g.tmask |= 1u << np;
+      g.pos[np++] = q;
+      ++ti;
+    } else if (others < 11u - K) {
+      g.pos[np++] = q;
+      ++others;
     }
   }
-  TileGeom g{};
-  g.Lc = Lc;
-  g.h = h;
-  g.tb = TB;
-  unsigned hi = 0;
-  for (int b = 0; b < K; ++b) {
-    if (qs[b] >= Lc) {
-      g.hbit[hi] = uint64_t{1} << qs[b];
-      g.outer[hi] = qs[b];
-      g.tpos[b] = Lc + hi;
-      ++hi;
-    } else {
-      g.tpos[b] = qs[b];
-    }
-  }
-  g.nouter = hi;
-  g.cbits_high = 0;
-  g.low_cmask = 0;
-  g.low_cvals = 0;
-  const unsigned groups = (1u << g.tb) >> K;
-  bool ok = hi == h && g.tb == g.Lc + g.h && g.tb <= n &&
-            groups % unsigned(Dm::G) == 0;
-  for (int b = 0; b < K; ++b) ok = ok && g.tpos[b] < g.tb;
-  for (int b = 1; b < K; ++b) ok = ok && g.tpos[b] > g.tpos[b - 1];
-  if (!ok) return QSIM_HIP_ERR_UNSUPPORTED;
-  g.ntiles = uint64_t{1} << (n - g.tb);
-  const size_t lds = (size_t{1} << g.tb) * sizeof(float2);
-  const uint64_t blocks = std::min<uint64_t>(g.ntiles, uint64_t{1} << 20);
+  if (np != 11 || ti != unsigned(K)) return QSIM_HIP_ERR_UNSUPPORTED;
+  const uint64_t blocks = std::max<uint64_t>(
+      1, std::min<uint64_t>(g.nsets, uint64_t(NumCUs()) * 4));
   PassTimerBegin();
-  hipLaunchKernelGGL(GateF16TileKernel<K>, dim3(unsigned(blocks)), dim3(256),
-                     lds, Stream(), s, g, f4, sm);
+  hipLaunchKernelGGL(GateF16SetKernel<K>, dim3(unsigned(blocks)), dim3(256), 0,
+                     Stream(), s, g, f4, sm);
   PassTimerEnd();
-  QHIP_LAUNCHED("GateF16TileKernel");
+  QHIP_LAUNCHED("GateF16SetKernel");
   return QSIM_HIP_OK;
 }
 

@@ -338,7 +338,8 @@ struct TrajTilePass {
 // of up to 14 qubits, 512 threads, one workgroup per CU (twice the bytes in
 // flight per trajectory, fewer passes, half as many trajectories at once).
 // One tile pass of the workgroup's trajectory: for every tile of the state,
-// stage it in LDS, run the pass's ops on it (split-BF16 MFMA), write it
+// stage it in LDS, run the pass's ops on it (split-BF16 MFMA with the
+// amplitudes in three parts), write it
 // back, with the next tile's loads in flight meanwhile.  The pass's op
 // records cycle through a two-slot ring in LDS as in the circuit kernel's
 // blocked passes (block_kernels.hip).  The tile sits at LDS offset 0 (no
@@ -394,7 +395,10 @@ __device__ __forceinline__ void TrajTileRun(float2* __restrict__ s,
       rd[q] = tab[q] ^ sj;
       wr[q] = tab[8 + q] ^ sj;
     }
-    ApplyTileOp(p.bpos[op], f, rd, wr, nbatch, wave, BlockShape<TB>::kWaves);
+    // Amplitudes in three bf16 parts (12 products per 32 groups): a
+    // trajectory's state stays within 1e-5 of FP32 arithmetic.
+    ApplyTileOp<true>(p.bpos[op], f, rd, wr, nbatch, wave,
+                      BlockShape<TB>::kWaves);
     uint4* slot = ring4 + ((k + 1) & 1u) * (kOpWords / 4);
     slot[threadIdx.x] = pre0;
     if (kRingQuads > 1) slot[threadIdx.x + NT] = pre1;

@@ -19,6 +19,7 @@ DRIVER_PATH = os.path.join(HERE, "lib", "libqsim_hip_driver.so")
 
 FUSER_MQUBIT = 0
 FUSER_LEARNED = 1
+FUSER_NONE = 2
 
 NUM_FEATURES = 18  # include/qsim_amd/learned_planner.h
 

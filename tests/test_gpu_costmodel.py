@@ -130,6 +130,15 @@ def test_small_model_trained_on_fresh_timings_plans_well():
     print(f"pick {cands[chosen]}: {t_pick:.2f} ms, fallback {t_fb:.2f} ms",
           flush=True)
     assert t_pick <= 1.25 * t_fb
+    # The pick's state against the unfused run.
+    cu = driver.Circuit(text)
+    cu.fuse(1, driver.FUSER_NONE)
+    su = ss.create(n)
+    ss.set_state_zero(su)
+    cu.run(su)
+    unf = su.to_numpy()
+    su.free()
+    assert np.linalg.norm(got - unf) / np.linalg.norm(unf) < 1e-5
     s64, _ = oracle.Circuit(text, np.float64).run()
     a = got[0::2] + 1j * got[1::2].astype(np.float64)
     b = s64[0::2] + 1j * s64[1::2]
@@ -149,15 +158,41 @@ def test_committed_model_gains_over_fallback(n, seed):
     chosen, cands = c.plan_learned(m)
     ss = qsim_amd.StateSpace(np.float32)
     st = ss.create(n)
+    su = ss.create(n)
     try:
         t_pick = _run_ms(c, ss, st)
         pred = cands[chosen]["predicted_ms"]
+        ss.set_state_zero(st)
+        c.run(st)
+        # The pick's state against the unfused run (every gate its own FP32
+        # pass), on the device: ||pick - unfused|| / ||unfused||.
+        cu = driver.Circuit(text)
+        cu.fuse(1, driver.FUSER_NONE)
+        ss.set_state_zero(su)
+        cu.run(su)
+        nu = ss.norm(su)
+        ss.multiply(-1.0, su)
+        ss.add(su, st)
+        err_u = float(np.sqrt(ss.norm(st) / nu))
         c.plan_learned(None)
         t_fb = _run_ms(c, ss, st)
+        # The best fixed choice among the blocked shapes the planner picks
+        # from most often, each planned for this circuit.
+        fixed = {}
+        for tb, lb in ((14, 4), (14, 5), (13, 5)):
+            c.fuse_blocked(tb, lb)
+            fixed[f"blocked {tb}/{lb}"] = _run_ms(c, ss, st)
     finally:
         st.free()
+        su.free()
+    best_fixed = min(fixed, key=fixed.get)
     print(f"{n}q seed {seed}: pick {cands[chosen]} measured {t_pick:.1f} ms "
           f"(predicted {pred:.1f}), fallback {t_fb:.1f} ms, gain "
-          f"{t_fb / t_pick:.2f}x", flush=True)
+          f"{t_fb / t_pick:.2f}x; best fixed {best_fixed} "
+          f"{fixed[best_fixed]:.1f} ms ({fixed[best_fixed] / t_pick:.3f}x); "
+          f"rel-L2 vs unfused {err_u:.2e}", flush=True)
     assert t_pick < t_fb
     assert abs(pred - t_pick) / t_pick < 0.5
+    # Never worse than the best fixed blocked shape beyond timing noise.
+    assert t_pick <= 1.05 * fixed[best_fixed]
+    assert err_u < 1e-5

@@ -111,11 +111,12 @@ def test_measurement_circuit_matches_oracle():
         assert rel_l2(st.to_numpy(), s_or) < 1e-12, seed
 
 
-@pytest.mark.parametrize("tile_bits,low_bits", [(13, 5), (12, 6), (10, 3)])
+@pytest.mark.parametrize("tile_bits,low_bits",
+                         [(14, 4), (14, 5), (13, 5), (12, 6), (10, 3)])
 def test_grid20_blocked_vs_oracle(grid20, tile_bits, low_bits):
-    # Cache-blocked passes (BF16x3 MFMA, FP64 fused matrices): the state
-    # meets F >= 1 - 1e-6 against the FP64 oracle and stays within 1e-4
-    # relative L2 of it.
+    # Cache-blocked passes (scaled FP16 MFMA, FP64 fused matrices): the
+    # state meets F >= 1 - 1e-6 against the FP64 oracle and stays within
+    # 1e-5 relative L2 of it and of the unfused FP32 run.
     text, s32, s64 = grid20
     c = driver.Circuit(text)
     npass = c.fuse_blocked(tile_bits, low_bits)
@@ -128,8 +129,19 @@ def test_grid20_blocked_vs_oracle(grid20, tile_bits, low_bits):
     err = rel_l2(got, s64)
     print(f"blocked {tile_bits}/{low_bits}: {npass} passes, rel-L2 vs FP64 "
           f"{err:.2e}, 1-F {1 - fid:.2e}", flush=True)
-    assert err < 1e-4
+    assert err < 1e-5
     assert fid > 1 - 1e-6
+    # Against the unfused complex64 run (every gate its own pass).
+    cu = driver.Circuit(text)
+    cu.fuse(1, driver.FUSER_NONE)
+    su = ss.create(20)
+    ss.set_state_zero(su)
+    cu.run(su)
+    err_u = rel_l2(got, su.to_numpy())
+    su.free()
+    print(f"blocked {tile_bits}/{low_bits}: rel-L2 vs unfused {err_u:.2e}",
+          flush=True)
+    assert err_u < 1e-5
     # The complex128 replay of the same partition is exact.
     ss64 = qsim_amd.StateSpace(np.float64)
     st64 = ss64.create(20)

@@ -1,12 +1,21 @@
 """Whole circuits captured as one HIP graph (qsim_driver_capture,
 qsim_hip_graph_*): every replay must reproduce the eager run bit for bit,
-with the program's own eager work and synchronisations between replays."""
+with the program's own eager work and synchronisations between replays, and
+the replayed state must match the CPU oracle's FP64 run of the same circuit
+(1e-5 relative L2 for complex64, 1e-12 for complex128)."""
 
 import numpy as np
 import pytest
 
+import oracle
 import qsim_amd
 from qsim_amd import circuits, driver
+
+
+def _oracle_rel_l2(text, got, dtype):
+    ref, _ = oracle.Circuit(text, np.float64).run()
+    got = got.astype(np.float64)
+    return float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
 
 pytestmark = pytest.mark.gpu
 
@@ -17,7 +26,7 @@ def _partitioned(text, how):
         c.fuse(4)
     elif how == "k6-f64":
         c.fuse(6, f64_matrices=True)
-        c.set_method(qsim_amd.METHOD_MFMA_BF16X3)
+        c.set_method(qsim_amd.METHOD_MFMA_FP16X2)
     elif how == "blocked":
         c.fuse_blocked(13, 5)
     elif how == "k2":
@@ -25,7 +34,7 @@ def _partitioned(text, how):
     return c
 
 
-def _replay_matches(c, n, dtype, replays=3):
+def _replay_matches(c, n, dtype, text, replays=3):
     ss = qsim_amd.StateSpace(dtype)
     ref = ss.create(n)
     ss.set_state_zero(ref)
@@ -44,15 +53,20 @@ def _replay_matches(c, n, dtype, replays=3):
         assert np.array_equal(got, want), f"replay {r}"
         # Eager work between replays (a reduction with its own sync).
         assert abs(ss.norm(st) - 1.0) < 1e-4
+    err = _oracle_rel_l2(text, got, dtype)
+    assert err < (1e-5 if dtype == np.float32 else 1e-12), err
     g.free()
+    st.free()
+    ref.free()
 
 
 @pytest.mark.parametrize("how", ["k4", "k6-f64", "blocked", "k2"])
 def test_graph_replays_the_circuit(how):
     n = 20
     print(f"graph replay n={n} partition={how}", flush=True)
-    c = _partitioned(circuits.grid_circuit(n, 14, seed=2), how)
-    _replay_matches(c, n, np.float32)
+    text = circuits.grid_circuit(n, 14, seed=2)
+    c = _partitioned(text, how)
+    _replay_matches(c, n, np.float32, text)
 
 
 def test_graph_complex128_and_controlled_gates():
@@ -60,8 +74,8 @@ def test_graph_complex128_and_controlled_gates():
     text = circuits.random_circuit(n, 200, seed=5, controlled_fraction=0.2)
     print(f"graph replay n={n} controlled, complex64 and complex128", flush=True)
     c = _partitioned(text, "k4")
-    _replay_matches(c, n, np.float32)
-    _replay_matches(c, n, np.float64)
+    _replay_matches(c, n, np.float32, text)
+    _replay_matches(c, n, np.float64, text)
 
 
 def test_graph_refuses_measurements():
@@ -76,8 +90,9 @@ def test_graph_refuses_measurements():
         c.capture(st)
     assert e.value.code == qsim_amd.ERR_UNSUPPORTED
     # The library is usable afterwards (no recording left open).
-    c2 = _partitioned(circuits.grid_circuit(n, 4, seed=0), "k4")
-    _replay_matches(c2, n, np.float32, replays=1)
+    text2 = circuits.grid_circuit(n, 4, seed=0)
+    c2 = _partitioned(text2, "k4")
+    _replay_matches(c2, n, np.float32, text2, replays=1)
 
 
 def test_graph_keeps_working_after_other_passes():

@@ -116,24 +116,24 @@ TILED = trajectories.TILED
 # 14-qubit tiles select the 512-thread shape; there the 12- and 13-qubit
 # states run partial tiles.
 @pytest.mark.parametrize("plan", [("tile", 9, 5), ("tile", 13, 5),
-                                  ("tile", 11, 3), ("tile", 14, 5),
-                                  ("tile", 14, 9)])
+                                  ("tile", 11, 3), ("tile", 14, 4),
+                                  ("tile", 14, 5), ("tile", 14, 9)])
 def test_tiled_kernel_matches_qtrajectory(tmp_path, plan):
     check_batch(tmp_path, [12, 14, 16], 8, 8, p=0.05, gamma=0.1,
                 threads=TILED, kmax=plan)
 
 
-@pytest.mark.parametrize("tile_bits", [13, 14])
-def test_tiled_mixed_18_20_22_batch(tmp_path, tile_bits):
+@pytest.mark.parametrize("tile_bits,low_bits", [(13, 5), (14, 5), (14, 4)])
+def test_tiled_mixed_18_20_22_batch(tmp_path, tile_bits, low_bits):
     check_batch(tmp_path, [18, 20, 22], 12, 4, p=1e-3, gamma=5e-3,
-                threads=TILED, kmax=("tile", tile_bits, 5))
+                threads=TILED, kmax=("tile", tile_bits, low_bits))
 
 
 def test_per_gate_kernels_run_tiled_plans():
     # The per-gate kernels skip the pass headers and apply the gates one by
     # one; a 6-qubit trajectory keeps fused gates in a tiled plan.  Same
-    # Kraus choices and <ZZ> as the tiled kernel; final states within the
-    # blocked passes' bar (1e-4 relative L2: split-BF16 against FP32
+    # Kraus choices and <ZZ> as the tiled kernel; final states within 1e-5
+    # relative L2 (split-BF16 with three-part amplitudes against FP32
     # arithmetic over some 30 gates).
     b = trajectories.TrajectoryBatch([6, 12, 16], 6, p_depol=0.05,
                                      gamma_damp=0.1)
@@ -147,7 +147,7 @@ def test_per_gate_kernels_run_tiled_plans():
         assert np.array_equal(b.kraus(t), k)
         assert np.max(np.abs(b.zz(t) - zz)) < 2e-5
         ref = b.state(t)
-        assert np.linalg.norm(st - ref) < 1e-4 * np.linalg.norm(ref)
+        assert np.linalg.norm(st - ref) < 1e-5 * np.linalg.norm(ref)
 
 
 @pytest.mark.parametrize("tile_bits", [10, 14])
