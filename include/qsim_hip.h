@@ -157,6 +157,20 @@ int qsim_hip_real_inner_product(int prec, const void* a, const void* b,
 int qsim_hip_norm(int prec, const void* state, unsigned num_qubits,
                   double* out);
 
+/* qsim_hip_inner_product (im != NULL) or qsim_hip_real_inner_product
+ * (im == NULL) without any device allocation: the partial sums go to a
+ * fixed device array of the library (2047 partial sums, 32 KiB, held until
+ * the process ends; every reduction of any state size fits it) instead of
+ * the grow-only scratch.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (graph capture, serving loops).  Needs an
+ * initialised library (qsim_hip_init or any earlier call); refuses with
+ * QSIM_HIP_ERR_ARG while the gate queue holds gates (qsim_hip_flush first,
+ * since running them could stage a pass).  Calls that overlap on one GPU
+ * (other processes, or graphs replayed beside it) race on the array. */
+int qsim_hip_inner_product_noalloc(int prec, const void* a, const void* b,
+                                   unsigned num_qubits, double* re,
+                                   double* im);
+
 /* Inverse-CDF sampling with caller-supplied draws: `draws` holds `num`
  * ascending values in [0, norm) (lib/util.h:67-85 order); out[j] = first index
  * k with draws[j] < sum_{i<=k} |s_i|^2 (FP64 prefix sums), or 2^n - 1 if none

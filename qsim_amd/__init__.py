@@ -83,6 +83,7 @@ SIGNATURES = [
     ("qsim_hip_inner_product", _i, [_i, _vp, _vp, _u, _f64p, _f64p]),
     ("qsim_hip_real_inner_product", _i, [_i, _vp, _vp, _u, _f64p]),
     ("qsim_hip_norm", _i, [_i, _vp, _u, _f64p]),
+    ("qsim_hip_inner_product_noalloc", _i, [_i, _vp, _vp, _u, _f64p, _f64p]),
     ("qsim_hip_sample", _i, [_i, _vp, _u, _f64p, _u64, _u64p]),
     ("qsim_hip_set_gate_queue", _i, [_u, _u]),
     ("qsim_hip_get_gate_queue", _i, [_vp, _vp, _vp, _vp]),

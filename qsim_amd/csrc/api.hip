@@ -114,6 +114,7 @@ static int InitDevice(int device) {
                                        hipEventDisableTiming));
   }
   for (int i = 0; i < 64; ++i) QHIP_CHECK(hipEventCreate(&g_dev.ev[i]));
+  QHIP_TRY(InitStaticScratch());
   g_dev.ready = true;
   return QSIM_HIP_OK;
 }
@@ -519,6 +520,32 @@ int qsim_hip_norm(int prec, const void* s, unsigned n, double* out) {
   return qsim_hip_real_inner_product(prec, s, s, n, out);
 }
 
+int qsim_hip_inner_product_noalloc(int prec, const void* a, const void* b,
+                                   unsigned n, double* re, double* im) {
+  QHIP_TRY(CheckStateArgs(prec, a, n));
+  QHIP_TRY(CheckStateArgs(prec, b, n));
+  if (re == nullptr) {
+    SetError("null result pointer");
+    return QSIM_HIP_ERR_ARG;
+  }
+  // Running queued gates could stage a blocked pass; this entry point
+  // leaves that to the caller (qsim_hip_flush) instead.
+  unsigned tb = 0;
+  uint64_t queued = 0;
+  QHIP_TRY(QueuedGates(&tb, &queued));
+  if (queued != 0) {
+    SetError("no-allocation inner product: the gate queue holds %llu gates; "
+             "call qsim_hip_flush first", (unsigned long long)queued);
+    return QSIM_HIP_ERR_ARG;
+  }
+  return QHIP_DISPATCH(
+      prec,
+      DotStatic<float>((const float2*)a, (const float2*)b, n, im == nullptr,
+                       re, im),
+      DotStatic<double>((const double2*)a, (const double2*)b, n,
+                        im == nullptr, re, im));
+}
+
 int qsim_hip_sample(int prec, const void* s, unsigned n, const double* draws,
                     uint64_t num, uint64_t* out) {
   QHIP_TRY(CheckState(prec, s, n));
@@ -707,6 +734,13 @@ static int FlushLocked() {
 int FlushGateQueue() {
   std::lock_guard<std::mutex> lock(g_queue_mu);
   return FlushLocked();
+}
+
+int QueuedGates(unsigned* tile_bits, uint64_t* gates) {
+  std::lock_guard<std::mutex> lock(g_queue_mu);
+  *tile_bits = g_queue.tb;
+  *gates = g_queue.ks.size();
+  return QSIM_HIP_OK;
 }
 
 // Queues the gate if the queue is on and takes it; *queued says whether it

@@ -55,6 +55,8 @@ int NumCUs();
 
 // Runs the gates the gate queue holds (qsim_hip_set_gate_queue).
 int FlushGateQueue();
+// The gate queue's tile qubits (0: off) and the number of gates it holds.
+int QueuedGates(unsigned* tile_bits, uint64_t* gates);
 
 // Op records an FP16 blocked pass of tile_bits tile qubits keeps resident in
 // LDS (longer programs stream the state once per such chunk).
@@ -181,6 +183,13 @@ template <typename FP> int Scale(CT<FP>* s, unsigned n, double a);
 template <typename FP>
 int Dot(const CT<FP>* a, const CT<FP>* b, unsigned n, bool real, double* re,
         double* im);
+// Dot on the fixed device scratch instead of the grow-only one: no device
+// allocation.  InitStaticScratch takes the scratch's device address (once,
+// at library initialisation, with the device current).
+template <typename FP>
+int DotStatic(const CT<FP>* a, const CT<FP>* b, unsigned n, bool real,
+              double* re, double* im);
+int InitStaticScratch();
 unsigned PartialNormCount(unsigned n);
 template <typename FP> int PartialNorms(const CT<FP>* s, unsigned n, double* out);
 template <typename FP>

@@ -259,13 +259,15 @@ SampleKernel(const C* __restrict__ s, uint64_t size, uint64_t sblock,
 template <typename FP>
 using C_t = typename Cplx<FP>::T;
 
+// Blocks of a reduction pass: at most kReduceBlocks partial sums.
+constexpr unsigned kReduceBlocks = 1024;
+
+// Partial sums (part + 1) and the final sum (part[0]) of a dot product
+// into `part`, which holds at least GridFor(size, kReduceBlocks) + 1 Dbl2.
 template <typename FP>
-int Reduce(const C_t<FP>* a, const C_t<FP>* b, uint64_t size, bool real,
-           double* re, double* im) {
-  std::lock_guard<std::mutex> lock(ScratchLock());
-  unsigned blocks = GridFor(size, 1024);
-  Dbl2* part = static_cast<Dbl2*>(DeviceScratch(sizeof(Dbl2) * (blocks + 1)));
-  if (part == nullptr) return QSIM_HIP_ERR_OOM;
+int ReduceInto(const C_t<FP>* a, const C_t<FP>* b, uint64_t size, bool real,
+               Dbl2* part, double* re, double* im) {
+  const unsigned blocks = GridFor(size, kReduceBlocks);
   if (real)
     hipLaunchKernelGGL((DotKernel<C_t<FP>, true>), dim3(blocks), dim3(kBS), 0,
                        Stream(), a, b, size, part + 1);
@@ -284,6 +286,28 @@ int Reduce(const C_t<FP>* a, const C_t<FP>* b, uint64_t size, bool real,
   if (im) *im = r.y;
   return QSIM_HIP_OK;
 }
+
+template <typename FP>
+int Reduce(const C_t<FP>* a, const C_t<FP>* b, uint64_t size, bool real,
+           double* re, double* im) {
+  std::lock_guard<std::mutex> lock(ScratchLock());
+  const unsigned blocks = GridFor(size, kReduceBlocks);
+  Dbl2* part = static_cast<Dbl2*>(DeviceScratch(sizeof(Dbl2) * (blocks + 1)));
+  if (part == nullptr) return QSIM_HIP_ERR_OOM;
+  return ReduceInto<FP>(a, b, size, real, part, re, im);
+}
+
+// ---- fixed scratch of the no-allocation reductions -------------------------
+//
+// The partial sums of DotStatic live in this array (one per device, held
+// until the process ends), as (re, im) pairs.  Its name is a host-side stand-in: host code
+// uses only the device address hipGetSymbolAddress gives for it
+// (g_static_scratch, taken once when the library is initialised).
+constexpr unsigned kStaticScratchDbl2 = 2048;
+static_assert(kStaticScratchDbl2 >= kReduceBlocks + 1, "reduction scratch");
+static __device__ __align__(256) double scratch[2 * kStaticScratchDbl2];
+Dbl2* g_static_scratch = nullptr;
+std::mutex g_static_scratch_mu;
 
 // Number of equal index ranges for partial norms.
 unsigned NumParts(unsigned n) {
@@ -368,6 +392,33 @@ template <typename FP>
 int Dot(const C_t<FP>* a, const C_t<FP>* b, unsigned n, bool real, double* re,
         double* im) {
   return Reduce<FP>(a, b, uint64_t{1} << n, real, re, im);
+}
+
+template <typename FP>
+int DotStatic(const C_t<FP>* a, const C_t<FP>* b, unsigned n, bool real,
+              double* re, double* im) {
+  const uint64_t size = uint64_t{1} << n;
+  const unsigned blocks = GridFor(size, kReduceBlocks);
+  if (g_static_scratch == nullptr) {
+    SetError("no-allocation reduction: the fixed scratch was not set up "
+             "(library not initialised)");
+    return QSIM_HIP_ERR_RUNTIME;
+  }
+  if (size_t(blocks) + 1 > kStaticScratchDbl2) {
+    SetError("no-allocation reduction: %u partial sums exceed the fixed "
+             "scratch of %u", blocks + 1, kStaticScratchDbl2 - 1);
+    return QSIM_HIP_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lock(g_static_scratch_mu);
+  return ReduceInto<FP>(a, b, size, real, g_static_scratch, re, im);
+}
+
+int InitStaticScratch() {
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return HipFail(e, "hipGetSymbolAddress(scratch)");
+  g_static_scratch = static_cast<Dbl2*>(p);
+  return QSIM_HIP_OK;
 }
 
 unsigned PartialNormCount(unsigned n) { return NumParts(n); }
@@ -511,6 +562,8 @@ int Gather(const C_t<FP>* s, const uint64_t* idx, uint64_t count,
   template int Scale<FP>(C_t<FP>*, unsigned, double);                          \
   template int Dot<FP>(const C_t<FP>*, const C_t<FP>*, unsigned, bool,         \
                        double*, double*);                                      \
+  template int DotStatic<FP>(const C_t<FP>*, const C_t<FP>*, unsigned, bool,   \
+                             double*, double*);                                \
   template int PartialNorms<FP>(const C_t<FP>*, unsigned, double*);            \
   template int FindMeasuredBits<FP>(const C_t<FP>*, unsigned, unsigned,        \
                                     double, uint64_t, uint64_t*);              \
