@@ -74,6 +74,12 @@ PATHS = {
     # the learned planner's pick (qsim_amd/data/cost_model.json)
     "learned": ("learned", 0, 0, 0, "f16"),
 }
+# The fusion sweep of the 30-qubit workload: qsim's fuser at
+# max_fused_size k with qsim's FP32 fused matrices on the FP32 kernels
+# (vector kernels below k = 4, exact-FP32 MFMA from k = 4).
+for _k in range(2, 7):
+    PATHS[f"fp32-k{_k}"] = ("mqubit", _k, False, 0, "f16")
+SWEEP_PATHS = [f"fp32-k{k}" for k in range(2, 7)]
 
 # HBM bytes per amplitude per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE
 # runs, scaled by what the counters read for a copy of known size (reads x 2,
@@ -112,8 +118,8 @@ def fuse(circ, name):
 
 def available_paths():
     from qsim_amd import costmodel
-    return [p for p in PATHS
-            if PATHS[p][0] != "learned" or os.path.exists(costmodel.MODEL_PATH)]
+    return [p for p in PATHS if p not in SWEEP_PATHS and
+            (PATHS[p][0] != "learned" or os.path.exists(costmodel.MODEL_PATH))]
 
 
 def kernel_family(circ, i, qubits, method, n):
@@ -378,6 +384,39 @@ def run_cpu_baseline(args, circuit_file):
     }
 
 
+def run_grid30(args):
+    """The 30-qubit workload (5 x 6 grid, 20 cycles, seed 0, 8 GiB of
+    complex64): qsim's fuser swept over max_fused_size 2..6 on the FP32
+    kernels, each pass's HBM GB/s, and qsim's own CPU path (QSimRunner with
+    MultiQubitGateFuser k = 4 on the host's widest SIMD simulator, every
+    core of the process's share) on the whole circuit.  Reported beside the
+    headline."""
+    from qsim_amd import circuits
+    n, cycles = 30, 20
+    text = circuits.grid_circuit(n, cycles, 0)
+    out = {"workload": "30q random grid circuit 5x6, 20 cycles, seed 0, "
+                       "complex64, qsim fuser k=2..6, FP32 kernels",
+           "per_k": {}}
+    for name in SWEEP_PATHS:
+        r = run_path(args, text, name)
+        out["per_k"][name.split("-k")[1]] = {
+            "passes": r["passes"], "circuit_s": r["circuit_s"],
+            "hbm_gbps_avg_pass": r["hbm_gbps_avg_pass"],
+            "frac_of_hbm_peak": r["hbm_gbps_avg_pass"] / HBM_PEAK_GBPS,
+            "kernels": r["kernels"], "final_norm": r["final_norm"]}
+    if not args.no_cpu:
+        with tempfile.TemporaryDirectory() as tmp:
+            cf = os.path.join(tmp, "grid30.txt")
+            with open(cf, "w") as f:
+                f.write(text)
+            try:
+                out["cpu_baseline"] = run_cpu_baseline(args, cf)
+            except Exception as e:  # reported, never silently replaced
+                log(f"30q cpu baseline failed: {e!r}")
+                out["cpu_baseline"] = {"value": None, "error": repr(e)}
+    return out
+
+
 def run_trajectories(args):
     """The batched noisy-trajectory engine on its own workload: 1024
     trajectories of the grid circuit at 18, 20 and 22 qubits (341-342 each),
@@ -543,6 +582,8 @@ def main():
     ap.add_argument("--trajectories", type=int, default=1024,
                     help="size of the trajectory batch reported beside the "
                          "headline (0: skip)")
+    ap.add_argument("--no-grid30", action="store_true",
+                    help="skip the 30-qubit fusion sweep and its CPU run")
     ap.add_argument("--no-small", action="store_true",
                     help="skip the 20-qubit eager vs captured-graph line")
     args = ap.parse_args()
@@ -604,6 +645,14 @@ def main():
                 log(f"cpu baseline failed: {e!r}")
                 cpu = {"value": None, "error": repr(e)}
 
+    grid30 = None
+    if not args.no_grid30:
+        try:
+            grid30 = run_grid30(args)
+        except Exception as e:  # reported beside the headline, never hidden
+            log(f"30q sweep failed: {e!r}")
+            grid30 = {"error": repr(e)}
+
     traj = None
     if args.trajectories > 0:
         try:
@@ -663,6 +712,7 @@ def main():
                   for p, r in results.items()},
         "roofline": b["roofline"],
         "cpu_baseline": cpu,
+        "grid30_fusion_sweep": grid30,
         "trajectory_batch": traj,
         "small_circuit": small,
     }

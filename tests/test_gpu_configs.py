@@ -21,6 +21,9 @@ and fSim(pi/2, pi/6) on alternating coupler patterns), 20 cycles, seed 0:
   path within 1e-5 relative L2 and 1 - F < 1e-6; split-BF16 k = 5/6 within
   1 - F < 1e-6 (its relative L2 is reported, not held to 1e-5: two-part
   bf16 amplitudes carry 2^-17 per pass).
+- 34 qubits (128 GiB of complex64) resident: a product state made by every
+  kernel family, checked amplitude by amplitude against FP64 products, and
+  returned to |0...0> by the gates' inverses.
 """
 
 import os
@@ -230,3 +233,94 @@ def test_grid28_paths_vs_cpu_fp64():
             st.free()
     finally:
         r.free()
+
+
+def _unitary(k, rng):
+    a = rng.standard_normal((1 << k, 1 << k)) + 1j * rng.standard_normal(
+        (1 << k, 1 << k))
+    q, r = np.linalg.qr(a)
+    return q * (np.diag(r) / np.abs(np.diag(r)))
+
+
+def _interleaved(u):
+    m = np.empty(2 * u.size)
+    m[0::2] = u.real.reshape(-1)
+    m[1::2] = u.imag.reshape(-1)
+    return m
+
+
+def test_34_qubits_resident():
+    # 34 qubits of complex64 (128 GiB) resident on one GPU.  Gates on
+    # disjoint qubit sets, through every kernel family (FP32 MFMA k = 4
+    # with targets up to qubit 27, FP16 k = 5 and 6, the vector kernel on
+    # qubits 32 and 33, and a blocked pass of two 4-qubit gates), make a
+    # product state: each amplitude is the product of one column-0 entry
+    # per gate, checked on gathered indices against FP64 products.  The
+    # gates' inverses then bring the state back to |0...0>.
+    n = 34
+    _, total, _ = qsim_amd.device_info()
+    if total < (5 << 30) * 32:
+        pytest.skip("needs 160 GiB of device memory")
+    rng = np.random.default_rng(34)
+    ss = qsim_amd.StateSpace(np.float32)
+    sim = qsim_amd.Simulator()
+    st = ss.create(n)
+    assert st is not None, "128 GiB state allocation failed"
+    plain = [([0, 1, 2, 3], qsim_amd.METHOD_AUTO),
+             ([8, 9, 10, 11], qsim_amd.METHOD_AUTO),
+             ([16, 17, 18, 19, 20], qsim_amd.METHOD_MFMA_FP16X2),
+             ([22, 23, 24, 25, 26, 27], qsim_amd.METHOD_MFMA_FP16X2),
+             ([32, 33], qsim_amd.METHOD_AUTO)]
+    blocked = [[4, 5, 6, 7], [12, 13, 14, 15]]
+    tile = [0, 1, 2, 3, 4, 5, 6, 7, 12, 13, 14, 15]
+    us = {tuple(q): _unitary(len(q), rng) for q, _ in plain}
+    us.update({tuple(q): _unitary(4, rng) for q in blocked})
+    try:
+        ss.set_state_zero(st)
+        for q, method in plain:
+            qsim_amd.set_gate_method(method)
+            sim.apply_gate(q, _interleaved(us[tuple(q)]), st)
+        qsim_amd.set_gate_method(qsim_amd.METHOD_AUTO)
+        sim.apply_gate_block(tile, [(q, _interleaved(us[tuple(q)]))
+                                    for q in blocked], st)
+        norm = ss.norm(st)
+        print(f"34q product state: norm {norm:.9f}", flush=True)
+        assert abs(norm - 1.0) < 1e-5
+        used = 0
+        for q in us:
+            for b in q:
+                used |= 1 << b
+        # Indices with every unused qubit 0 (nonzero amplitudes) and some
+        # with an unused qubit set (exactly zero).
+        idx = rng.integers(0, 1 << n, size=4000, dtype=np.uint64)
+        idx[:3000] &= np.uint64(used)
+        got = ss.get_ampls(st, idx).astype(np.complex128)
+        want = np.ones(idx.size, np.complex128)
+        for q, u in us.items():
+            col = np.zeros(idx.size, np.int64)
+            for j, b in enumerate(q):
+                col |= ((idx >> np.uint64(b)) & np.uint64(1)).astype(
+                    np.int64) << j
+            want *= u[col, 0]
+        want[(idx & ~np.uint64(used)) != 0] = 0.0
+        err = np.max(np.abs(got - want))
+        scale = np.max(np.abs(want))
+        print(f"34q gathered amplitudes: max error {err:.2e} (largest "
+              f"{scale:.2e})", flush=True)
+        assert err < 1e-5 * scale
+        # Inverses, in reverse order.
+        sim.apply_gate_block(tile, [(q, _interleaved(us[tuple(q)].conj().T))
+                                    for q in reversed(blocked)], st)
+        for q, method in reversed(plain):
+            qsim_amd.set_gate_method(method)
+            sim.apply_gate(q, _interleaved(us[tuple(q)].conj().T), st)
+        qsim_amd.set_gate_method(qsim_amd.METHOD_AUTO)
+        a0 = complex(ss.get_ampls(st, np.array([0], np.uint64))[0])
+        p0 = abs(a0) ** 2
+        norm = ss.norm(st)
+        print(f"34q after the inverses: |<0|psi>|^2 {p0:.9f}, norm "
+              f"{norm:.9f}", flush=True)
+        assert abs(p0 - 1.0) < 1e-5 and abs(norm - 1.0) < 1e-5
+    finally:
+        qsim_amd.set_gate_method(qsim_amd.METHOD_AUTO)
+        st.free()
