@@ -119,7 +119,8 @@ GateBlockKernel(float2* __restrict__ s, BlockGeom g, BlockProgram p,
         rd[q] = tab[q] ^ sj;
         wr[q] = tab[8 + q] ^ sj;
       }
-      ApplyTileOp<ACC>(p.bpos[op], f, rd, wr, nbatch, wave,
+      ApplyTileOp<ACC>(reinterpret_cast<const uint8_t*>(cur + kBposWord), f,
+                       rd, wr, nbatch, wave,
                        BlockShape<TB>::kWaves);
       uint4* slot = ring4 + ((k + 1) & 1u) * (kOpWords / 4);
       slot[threadIdx.x] = pre0;
@@ -244,7 +245,8 @@ GateBlockF16Kernel(float2* __restrict__ s, BlockGeom g, BlockProgram p,
         rd[q] = tab[q] ^ sj;
         wr[q] = tab[8 + q] ^ sj;
       }
-      ApplyTileOpF16(p.bpos[op], f, rd, wr, nbatch, wave, NW);
+      ApplyTileOpF16(reinterpret_cast<const uint8_t*>(r + kF16BposOff), f,
+                         rd, wr, nbatch, wave, NW);
       __syncthreads();
     }
     io.template StoreScaled<16, FULL>(s, base, g, lds, e);
@@ -565,12 +567,19 @@ int PrepareGateBlock(unsigned n, const unsigned* tile, unsigned nt,
           uint32_t(sizeof(float2)) * HostTileSwz(Deposit(j, lane, 5));
     }
     for (unsigned b = 0; b < kBlockMaxBits - 9; ++b) p.bpos[i][b] = uint8_t(batch[b]);
+    uint32_t bw[2] = {0, 0};
+    std::memcpy(bw, p.bpos[i], kBlockMaxBits - 9);
     if (arith == kBlockArithF16) {
       uint32_t r32[kF16RecWords];
       BuildRecF16(u16.data(), tab, r32);
+      r32[kF16BposOff] = bw[0];
+      r32[kF16BposOff + 1] = bw[1];
+      r32[kF16BposOff + 2] = 0;
+      r32[kF16BposOff + 3] = 0;
       std::memcpy(rec, r32, sizeof(r32));
     } else {
       std::memcpy(rec + 2 * size_t(kFragWords), tab, sizeof(tab));
+      std::memcpy(rec + 2 * size_t(kBposWord), bw, sizeof(bw));
     }
   }
   if (arith == kBlockArithF16) {

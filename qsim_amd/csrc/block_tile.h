@@ -46,12 +46,17 @@ constexpr size_t kOpFrag = 2 * 3 * 512;
 // tables -- LDS byte offsets, relative to a group's slot, of the members a
 // lane reads (8) and writes (8), for lane half 0 then 1 (32 words); the
 // group table -- LDS byte offset of lane j's group within a batch, j < 32
-// (32 words); padded to 8 KiB.
+// (32 words); the op's five batch-bit positions (one byte each, 2 words);
+// padded to 8 KiB.  The positions travel in the record so that an op reads
+// them from LDS: read from the kernel arguments (bytes at a run-time index)
+// they would be vector memory loads, and waiting for them would wait for
+// every tile load issued before them.
 constexpr int kFragWords = int(kOpFrag / 2);
 constexpr int kTableWords = 32;
 constexpr int kLaneWords = 32;
+constexpr int kBposWord = kFragWords + kTableWords + kLaneWords;
 constexpr int kOpWords = 2048;
-static_assert(kFragWords + kTableWords + kLaneWords <= kOpWords, "op record");
+static_assert(kBposWord + 2 <= kOpWords, "op record");
 template <int TB>
 constexpr size_t BlockLds() {
   return BlockShape<TB>::kTileBytes + 2 * kOpWords * sizeof(uint32_t);
@@ -181,7 +186,10 @@ __device__ __forceinline__ void ApplyTileOp(const uint8_t* bp,
                                             const uint32_t (&wr)[8],
                                             unsigned nbatch, unsigned wave,
                                             unsigned nwaves) {
-  const unsigned b[5] = {bp[0], bp[1], bp[2], bp[3], bp[4]};
+  unsigned b[5];
+#pragma unroll
+  for (int i = 0; i < 5; ++i)
+    b[i] = unsigned(__builtin_amdgcn_readfirstlane(int(bp[i])));
   if (ACCURATE) {
     for (unsigned bt = wave; bt < nbatch; bt += nwaves) {
       const unsigned sp0 = BatchSlot(bt, b);
@@ -240,10 +248,12 @@ typedef _Float16 v2h __attribute__((ext_vector_type(2)));
 
 // FP16 op record, 32-bit words: the 16 x 16 complex matrix as packed
 // (re, im) fp16 pairs at word 16 r + c, hi part then lo part; the member
-// tables and the group table as in the bf16 record.
+// tables, the group table and the batch-bit positions as in the bf16
+// record; padded to a multiple of 4 words.
 constexpr int kF16MatWords = 256;
 constexpr int kF16TabOff = 2 * kF16MatWords;
-constexpr int kF16RecWords = kF16TabOff + kTableWords + kLaneWords;  // 576
+constexpr int kF16BposOff = kF16TabOff + kTableWords + kLaneWords;
+constexpr int kF16RecWords = kF16BposOff + 4;  // 580
 static_assert(kF16RecWords % 4 == 0, "records are copied as uint4");
 
 // LDS of an FP16 blocked-pass workgroup: the tile, 16 floats for the
@@ -330,7 +340,10 @@ __device__ __forceinline__ void ApplyTileOpF16(const uint8_t* bp,
                                                const uint32_t (&wr)[8],
                                                unsigned nbatch, unsigned wave,
                                                unsigned nwaves) {
-  const unsigned b[5] = {bp[0], bp[1], bp[2], bp[3], bp[4]};
+  unsigned b[5];
+#pragma unroll
+  for (int i = 0; i < 5; ++i)
+    b[i] = unsigned(__builtin_amdgcn_readfirstlane(int(bp[i])));
   unsigned bt = wave;
   for (; bt + nwaves < nbatch; bt += 2 * nwaves) {
     const unsigned sp0 = BatchSlot(bt, b);

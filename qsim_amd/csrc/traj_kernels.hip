@@ -397,8 +397,8 @@ __device__ __forceinline__ void TrajTileRun(float2* __restrict__ s,
     }
     // Amplitudes in three bf16 parts (12 products per 32 groups): a
     // trajectory's state stays within 1e-5 of FP32 arithmetic.
-    ApplyTileOp<true>(p.bpos[op], f, rd, wr, nbatch, wave,
-                      BlockShape<TB>::kWaves);
+    ApplyTileOp<true>(reinterpret_cast<const uint8_t*>(cur + kBposWord), f,
+                      rd, wr, nbatch, wave, BlockShape<TB>::kWaves);
     uint4* slot = ring4 + ((k + 1) & 1u) * (kOpWords / 4);
     slot[threadIdx.x] = pre0;
     if (kRingQuads > 1) slot[threadIdx.x + NT] = pre1;
