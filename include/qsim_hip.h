@@ -360,17 +360,27 @@ int qsim_hip_pass_timer_stop(float* ms, unsigned max_count, unsigned* count);
  * Many quantum trajectories (of possibly different qubit counts) run in ONE
  * kernel launch, one workgroup per trajectory, each with its own state in a
  * device pool and its own operation stream: fused gates (FP32 matrices, the
- * Kraus operators already sampled on the host folded in) and amplitude-
- * damping decisions that need the state's norm (taken on the device, in
- * FP64).  At the end every trajectory writes <Z_a Z_b> for its qubit pairs
- * (FP64).  Replaces the serial loop of QuantumTrajectorySimulator::RunBatch
- * / RunIteration (lib/qtrajectory.h:112-387), with the same sampling rule:
- * a channel's Kraus operator is chosen from the draw against probability
- * bounds, and against the norms of the updated state when the bounds do not
- * decide (lib/qtrajectory.h:296-356). */
+ * Kraus operators already sampled on the host folded in) and channel
+ * decisions that need the state (taken on the device, in FP64).  At the end
+ * every trajectory writes <Z_a Z_b> for its qubit pairs (FP64).  Replaces
+ * the serial loop of QuantumTrajectorySimulator::RunBatch / RunIteration
+ * (lib/qtrajectory.h:112-387), with the same sampling rule: a channel's
+ * Kraus operator is chosen from the draw against probability bounds, and
+ * against the probabilities <psi|K^dagger K|psi> of the (normalised) state
+ * when the bounds do not decide (lib/qtrajectory.h:322-370). */
 enum {
   QSIM_HIP_TRAJ_GATE = 0,
-  QSIM_HIP_TRAJ_DAMP_DECIDE = 1,
+  /* A channel the bounds did not decide, followed by its k Kraus operators
+   * (QSIM_HIP_TRAJ_KRAUS, 1 <= k <= 16, in the channel's order): q[0..moff2)
+   * are the channel's qubits (moff2 = 1 or 2, ascending), r the draw, cp the
+   * sum of the operators' probability bounds, slot the decision record
+   * index, pad bit 0 = normalise the state first (the trajectory has
+   * applied a non-unitary operator).  Walks the operators as RunIteration
+   * does (lib/qtrajectory.h:330-370): unitary ones are skipped; for the
+   * others cp += p_i - bound_i with p_i = <psi|K_i^dagger K_i|psi> (FP64
+   * sums), the first with r < cp is applied -- or, at the last operator,
+   * the likeliest one -- scaled to the normalised state. */
+  QSIM_HIP_TRAJ_CHANNEL = 1,
   /* The header of a tile pass over the k gate operations that follow it
    * (1 <= k <= 32): q[0] | q[1] << 32 is the mask of its tile qubits (9..14
    * of them, starting with qubits 0, 1, 2), and every gate lies inside the
@@ -378,7 +388,12 @@ enum {
    * state (tiles staged in LDS, split-BF16 MFMA, as
    * qsim_hip_apply_gate_block does); the per-gate kernels ignore the header
    * and apply the gates one by one.  moff is not read. */
-  QSIM_HIP_TRAJ_TILE_PASS = 2
+  QSIM_HIP_TRAJ_TILE_PASS = 2,
+  /* A Kraus operator of the channel before it: K on q[0..k) (k = 0: the
+   * identity) at matrix offset moff, K^dagger K over the channel's qubits at
+   * moff2 (not read for a unitary operator), r its probability bound, pad
+   * bit 0 = unitary. */
+  QSIM_HIP_TRAJ_KRAUS = 3
 };
 /* qsim_hip_traj_run's `threads` value for the tiled kernel: 256 threads
  * per trajectory, or 512 when a tile pass has 14 qubits. */
@@ -386,15 +401,16 @@ enum { QSIM_HIP_TRAJ_TILED = 1 };
 
 typedef struct {
   uint32_t kind;   /* QSIM_HIP_TRAJ_* */
-  uint32_t k;      /* gate: qubit count (1..4) */
-  uint32_t q[4];   /* gate: ascending qubits; decision: q[0] */
-  uint32_t moff;   /* gate: matrix offset (floats); decision: K0 (2x2) */
-  uint32_t moff2;  /* decision: K1 (2x2) */
-  float r;         /* decision: the trajectory's draw for the channel */
-  float kdk1;      /* decision: (K0^dagger K0)[1][1] */
-  uint32_t slot;   /* decision: index in the decision record */
-  uint32_t pad;    /* decision: bit 0 = normalise the state first (the
-                      trajectory has applied a non-unitary operator) */
+  uint32_t k;      /* gate, Kraus: qubit count; channel, tile pass: number of
+                      operations that follow it */
+  uint32_t q[4];   /* gate, Kraus: ascending qubits; channel: its qubits;
+                      tile pass: tile-qubit mask */
+  uint32_t moff;   /* gate, Kraus: matrix offset (floats) */
+  uint32_t moff2;  /* Kraus: K^dagger K offset; channel: qubit count */
+  double r;        /* channel: the draw; Kraus: probability bound */
+  double cp;       /* channel: sum of the probability bounds */
+  uint32_t slot;   /* channel: index in the decision record */
+  uint32_t pad;    /* channel: bit 0 normalise first; Kraus: bit 0 unitary */
 } qsim_hip_traj_op;
 
 typedef struct {
@@ -421,7 +437,8 @@ int qsim_hip_traj_prepare(const qsim_hip_traj_desc* trajs, unsigned num_traj,
  * Waits for completion. */
 int qsim_hip_traj_run(void* handle, unsigned threads, int use_graph);
 /* <Z_a Z_b> of every pair (FP64, pair order of the batch) and the decision
- * record (0: K0, 1: K1, -1: not reached). */
+ * record (the chosen Kraus operator's index in its channel; -1: none chosen
+ * or not reached). */
 int qsim_hip_traj_results(void* handle, double* zz, uint32_t num_pairs,
                           int32_t* decisions, uint32_t num_decisions);
 /* Final state of trajectory t (complex64, 2^n pairs) to host memory,

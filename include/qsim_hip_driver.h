@@ -111,20 +111,42 @@ int qsim_driver_plan_learned(void* circuit, void* model,
 void qsim_driver_set_cost_model(void* model);
 
 /* ---- batched noisy trajectories (qsim_amd/apps/trajectories.cc) ---------
- * Noisy versions of qsim circuits on rows x cols grids: a depolarising
- * channel (p_depol) on each qubit of every gate after it, an amplitude-
- * damping channel (gamma_damp) on every qubit after each cycle (gate times
- * 2c, 2c + 1).  Each trajectory replays RunIteration's sampling
- * (lib/qtrajectory.h:240-387) from mt19937(rep) on the host where the
- * probability bounds decide it; the rest is decided on the device.  All
- * trajectories then run in one launch (qsim_hip_traj_run).  Replaces
- * QuantumTrajectorySimulator::RunBatch (lib/qtrajectory.h:112-175) over
- * reps; results are <Z_a Z_b> on grid neighbours (horizontal pairs then
- * vertical ones, row by row) and the Kraus index of every channel
- * (collect_kop_stat order without the gates' zeros). */
+ * Noisy circuits on rows x cols grids of qubits: gates and qsim channels
+ * (Channel, lib/channel.h:30-125).  Each trajectory replays RunIteration's
+ * sampling (lib/qtrajectory.h:240-387) from mt19937(rep) on the host where
+ * the probability bounds decide it; the rest is decided on the device
+ * (QSIM_HIP_TRAJ_CHANNEL; channels on at most 2 qubits with at most 16
+ * Kraus operators).  All trajectories then run in one launch
+ * (qsim_hip_traj_run).  Replaces QuantumTrajectorySimulator::RunBatch
+ * (lib/qtrajectory.h:112-175) over reps; results are <Z_a Z_b> on grid
+ * neighbours (horizontal pairs then vertical ones, row by row) and the
+ * Kraus index of every channel (collect_kop_stat order without the gates'
+ * zeros; -1 where a device-decided channel chose none).
+ *
+ * qsim_traj_create: qsim circuits made noisy by one fixed rule -- a
+ * depolarising channel (p_depol) on each qubit of every gate after it, an
+ * amplitude-damping channel (gamma_damp) on every qubit after each cycle
+ * (gate times 2c, 2c + 1).  NULL on a parse error. */
 void* qsim_traj_create(const char* const* circuits, unsigned num_circuits,
                        const unsigned* rows, const unsigned* cols,
                        double p_depol, double gamma_damp);
+/* Noisy circuits given as text: qsim's circuit format
+ * (lib/circuit_qsim_parser.h) with channel lines among the gate lines, in
+ * the order the operations apply:
+ *   <time> channel <name> <qubit> <params>, name one of depolarize p,
+ *       asymmetric_depolarize px py pz, amplitude_damp gamma,
+ *       generalized_amplitude_damp p gamma, phase_damp gamma, phase_flip p,
+ *       bit_flip p, reset (built by qsim's factories, lib/channels_cirq.h);
+ *   <time> channel kraus <nq> <qubits, ascending> <number of operators>,
+ *       then per operator <unitary 0|1> <probability bound> and its matrix
+ *       (2 4^nq numbers, row-major complex; bit j of the index <-> the j-th
+ *       qubit) -- K^dagger K as KrausOperator::CalculateKdKMatrix forms it.
+ * Gates and channels keep their times; gates act on at most 4 qubits.  A
+ * channel beyond the device's limits plans only where its bounds decide
+ * (QSIM_HIP_ERR_UNSUPPORTED from qsim_traj_plan otherwise).  NULL on a
+ * parse error. */
+void* qsim_traj_create_noisy(const char* const* texts, unsigned num_circuits,
+                             const unsigned* rows, const unsigned* cols);
 unsigned qsim_traj_num_channels(void* batch, unsigned circuit);
 /* Trajectory t runs circuit circuit_of[t] with repetition id reps[t]; runs
  * of deferred gates are fused to at most max_fused_size (1..4) qubits. */

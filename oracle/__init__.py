@@ -312,10 +312,13 @@ def reference_draws(num, seed, max_value, path):
 
 def run_reference_trajectories(circuit_file, rows, cols, rep0, rep1,
                                p_depol=1e-3, gamma=5e-3, max_fused=4,
-                               threads=1, timeout=3600, widest_simd=False):
+                               threads=1, timeout=3600, widest_simd=False,
+                               noisy_text=False):
     """qsim's QuantumTrajectorySimulator on the noisy grid circuit
-    (oracle/_ref/qsim_ref -T): per repetition the Kraus index of every
-    channel and <Z_a Z_b> of the grid-neighbour pairs; plus a summary."""
+    (oracle/_ref/qsim_ref -T; with noisy_text, the file is noisy-circuit
+    text of gates and channels, -N): per repetition the Kraus index of
+    every channel and <Z_a Z_b> of the grid-neighbour pairs; plus a
+    summary."""
     import json
     exe = REF_BIN
     if widest_simd and os.path.exists(REF_BIN_AVX512) and host_has_avx512():
@@ -325,6 +328,8 @@ def run_reference_trajectories(circuit_file, rows, cols, rep0, rep1,
     cmd = [exe, "-c", circuit_file, "-T", f"{rep0}:{rep1}", "-g",
            f"{rows}:{cols}", "-p", repr(float(p_depol)), "-G",
            repr(float(gamma)), "-f", str(max_fused), "-t", str(threads)]
+    if noisy_text:
+        cmd.append("-N")
     out = subprocess.run(cmd, check=True, capture_output=True, text=True,
                          timeout=timeout)
     lines = [json.loads(s) for s in out.stdout.strip().splitlines()]

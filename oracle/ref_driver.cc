@@ -10,7 +10,8 @@
 //            [-b BITSTRINGS.bin] [-d MAXTIME] [-P PASSES]
 //   qsim_ref -R NUM -x SEED -M MAX -b DRAWS.bin     (GenerateRandomValues)
 //   qsim_ref -c FILE -T R0:R1 -g ROWS:COLS [-p P] [-G GAMMA] [-f MAX_FUSED]
-//            [-t THREADS]       (noisy trajectories, QuantumTrajectorySimulator)
+//            [-t THREADS] [-N]  (noisy trajectories, QuantumTrajectorySimulator;
+//                                -N: FILE is noisy-circuit text, see NoisyText)
 //
 // The simulation is QSimRunner::Run with MultiQubitGateFuser
 // (lib/run_qsim.h:225-289, lib/fuser_mqubit.h:185-526) on the state of the
@@ -25,6 +26,8 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <fstream>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -72,6 +75,7 @@ struct Options {
   uint64_t rep0 = 0, rep1 = 0;
   unsigned rows = 0, cols = 0;
   double p_depol = 1e-3, gamma = 5e-3;
+  bool noisy_text = false;
 };
 
 template <typename Sim, typename FP>
@@ -228,6 +232,100 @@ qsim::Circuit<qsim::Operation<float>> NoisyGrid(
   return nc;
 }
 
+// Noisy-circuit text: qsim circuit lines, and lines "<time> channel <name>
+// <qubit> <params>" (name: depolarize, asymmetric_depolarize,
+// amplitude_damp, generalized_amplitude_damp, phase_damp, phase_flip,
+// bit_flip, reset; built by qsim's factories, lib/channels_cirq.h) or
+// "<time> channel kraus <nq> <qubits> <nk> {<unitary> <prob> <matrix>}".
+// Operations in file order, gates and channels with their given times.
+bool NoisyText(const std::string& path,
+               qsim::Circuit<qsim::Operation<float>>& nc) {
+  using namespace qsim;
+  std::ifstream f(path);
+  if (!f) return false;
+  std::string line;
+  unsigned n = 0;
+  while (std::getline(f, line)) {
+    std::istringstream in(line);
+    if (in >> n) break;
+  }
+  if (n == 0) return false;
+  nc.num_qubits = n;
+  while (std::getline(f, line)) {
+    std::istringstream in(line);
+    unsigned time;
+    std::string word;
+    if (!(in >> time >> word)) continue;
+    if (word != "channel") {
+      Circuit<Operation<float>> one;
+      std::stringstream gs;
+      gs << n << "\n" << line << "\n";
+      if (!CircuitQsimParser<IO>::FromStream(~0u, "line", gs, one)) {
+        return false;
+      }
+      for (auto& op : one.ops) nc.ops.push_back(std::move(op));
+      continue;
+    }
+    std::string name;
+    in >> name;
+    if (name == "kraus") {
+      unsigned nq, nk;
+      in >> nq;
+      std::vector<unsigned> qs(nq);
+      for (auto& q : qs) in >> q;
+      in >> nk;
+      Channel<float> ch{{kChannel, time, qs}, {}};
+      for (unsigned i = 0; i < nk; ++i) {
+        int unitary;
+        double prob;
+        in >> unitary >> prob;
+        Matrix<float> m(size_t{2} << (2 * nq));
+        for (auto& v : m) {
+          double x;
+          in >> x;
+          v = float(x);
+        }
+        KrausOperator<float> kop;
+        kop.unitary = unitary != 0;
+        kop.prob = prob;
+        kop.ops.push_back(
+            CreateGate<Gate<float>, Cirq::MatrixGate<float>>(time, qs,
+                                                              std::move(m)));
+        kop.CalculateKdKMatrix();
+        ch.kops.push_back(std::move(kop));
+      }
+      if (!in) return false;
+      nc.ops.push_back(std::move(ch));
+      continue;
+    }
+    unsigned q;
+    double a = 0, b = 0, c = 0;
+    in >> q;
+    if (name == "depolarize" && (in >> a)) {
+      nc.ops.push_back(Cirq::DepolarizingChannel<float>::Create(time, q, a));
+    } else if (name == "asymmetric_depolarize" && (in >> a >> b >> c)) {
+      nc.ops.push_back(
+          Cirq::AsymmetricDepolarizingChannel<float>::Create(time, q, a, b, c));
+    } else if (name == "amplitude_damp" && (in >> a)) {
+      nc.ops.push_back(Cirq::AmplitudeDampingChannel<float>::Create(time, q, a));
+    } else if (name == "generalized_amplitude_damp" && (in >> a >> b)) {
+      nc.ops.push_back(
+          Cirq::GeneralizedAmplitudeDampingChannel<float>::Create(time, q, a, b));
+    } else if (name == "phase_damp" && (in >> a)) {
+      nc.ops.push_back(Cirq::PhaseDampingChannel<float>::Create(time, q, a));
+    } else if (name == "phase_flip" && (in >> a)) {
+      nc.ops.push_back(Cirq::PhaseFlipChannel<float>::Create(time, q, a));
+    } else if (name == "bit_flip" && (in >> a)) {
+      nc.ops.push_back(Cirq::BitFlipChannel<float>::Create(time, q, a));
+    } else if (name == "reset") {
+      nc.ops.push_back(Cirq::ResetChannel<float>::Create(time, q));
+    } else {
+      return false;
+    }
+  }
+  return true;
+}
+
 // qsim's QuantumTrajectorySimulator (lib/qtrajectory.h) on SimulatorAVX for
 // repetitions [r0, r1): one JSON line per trajectory with the Kraus index of
 // every channel (collect_kop_stat, gates' zeros dropped) and <Z_a Z_b> of the
@@ -238,14 +336,19 @@ int RunTrajectories(const Options& opt) {
   using Fac = Factory<Sim, float>;
   using Runner = QSimRunner<IO, MultiQubitGateFuser<IO>, Fac>;
   using QT = QuantumTrajectorySimulator<IO, Runner>;
-  Circuit<Operation<float>> circuit;
-  if (!CircuitQsimParser<IOFile>::FromFile(opt.maxtime, opt.circuit_file,
-                                           circuit)) {
-    return 1;
+  Circuit<Operation<float>> circuit, nc;
+  if (opt.noisy_text) {
+    if (!NoisyText(opt.circuit_file, nc)) return 1;
+    circuit.num_qubits = nc.num_qubits;
+  } else {
+    if (!CircuitQsimParser<IOFile>::FromFile(opt.maxtime, opt.circuit_file,
+                                             circuit)) {
+      return 1;
+    }
+    nc = NoisyGrid(circuit, opt.p_depol, opt.gamma);
   }
   const unsigned rows = opt.rows, cols = opt.cols;
   if (rows * cols != circuit.num_qubits) return 1;
-  auto nc = NoisyGrid(circuit, opt.p_depol, opt.gamma);
   std::vector<char> is_channel;
   for (const auto& op : nc.ops) {
     is_channel.push_back(OpGetAlternative<Channel<float>>(op) != nullptr);
@@ -300,8 +403,9 @@ int RunTrajectories(const Options& opt) {
 int main(int argc, char* argv[]) {
   Options opt;
   int k;
-  while ((k = getopt(argc, argv, "c:m:f:t:s:o:n:x:b:d:R:M:P:T:g:p:G:")) != -1) {
+  while ((k = getopt(argc, argv, "c:m:f:t:s:o:n:x:b:d:R:M:P:T:g:p:G:N")) != -1) {
     switch (k) {
+      case 'N': opt.noisy_text = true; break;
       case 'T':
         opt.trajectories = true;
         if (std::sscanf(optarg, "%llu:%llu",

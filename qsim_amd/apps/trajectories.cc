@@ -14,16 +14,25 @@
 // the state itself.  All streams then run in one launch
 // (qsim_hip_traj_run).
 //
-// The noisy circuit is built from a qsim circuit by one fixed rule: after
-// every gate, a depolarising channel (DepolarizingChannel,
-// lib/channels_cirq.h:127-200) on each of its qubits; after the last gate of
-// every cycle (gate times 2c and 2c + 1), an amplitude-damping channel
-// (AmplitudeDampingChannel, lib/channels_cirq.h:274-300) on every qubit.
+// A noisy circuit is a sequence of gates and qsim channels (Channel,
+// lib/channel.h:30-125), either
+//  - built from a qsim circuit by one fixed rule (qsim_traj_create): after
+//    every gate, a depolarising channel (DepolarizingChannel,
+//    lib/channels_cirq.h:127-200) on each of its qubits; after the last gate
+//    of every cycle (gate times 2c and 2c + 1), an amplitude-damping channel
+//    (AmplitudeDampingChannel, lib/channels_cirq.h:274-300) on every qubit;
+//  - or read from noisy-circuit text (qsim_traj_create_noisy): qsim's
+//    circuit lines, and channel lines built by qsim's own channel factories
+//    (lib/channels_cirq.h) or given as explicit Kraus operators.
+// Any channel's operators go through the same sampling; those the bounds
+// do not decide are decided on the device (QSIM_HIP_TRAJ_CHANNEL), for
+// channels on at most 2 qubits with at most 16 operators.
 
 #include <omp.h>
 
 #include <algorithm>
 #include <atomic>
+#include <set>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -53,16 +62,127 @@ using namespace qsim;
 struct NoisyOp {
   int kind;  // 0 gate (index into the circuit), 1 channel (index into chans)
   size_t index;
+  unsigned time;  // gate: the time of its deferred copy
+};
+
+// Most Kraus operators and qubits of a channel that the device decides
+// (traj_kernels.hip).
+constexpr unsigned kMaxDeviceKraus = 16;
+constexpr unsigned kMaxDeviceChannelQubits = 2;
+
+// A channel in the form the device decides it in: its qubits and, per Kraus
+// operator, K on the operator's own qubits (the product of its gates, FP64,
+// rounded to FP32; empty for the identity) and K^dagger K over the
+// channel's qubits (FP32; empty for a unitary operator).
+struct DeviceChannel {
+  bool ok = false;  // within the device's limits
+  std::vector<unsigned> qubits;
+  std::vector<std::vector<unsigned>> kq;
+  std::vector<std::vector<float>> k, kdk;
 };
 
 struct NoisyCircuit {
   unsigned num_qubits = 0;
   Circuit<Operation<double>> circuit;
   std::vector<Channel<double>> chans;
+  std::vector<DeviceChannel> dchans;  // per channel
   std::vector<NoisyOp> ops;
   std::vector<uint32_t> pairs;  // grid-neighbour pairs (a, b)
   unsigned num_channels = 0;
 };
+
+// K^dagger K of `kop` (over kop.qubits) as a matrix over `qubits` (which
+// contain them): the tensor product with the identity on the others.
+std::vector<double> ExpandKdK(const KrausOperator<double>& kop,
+                              const std::vector<unsigned>& qubits) {
+  const unsigned nq = qubits.size(), D = 1u << nq;
+  unsigned pos[kMaxDeviceChannelQubits] = {0, 0};
+  for (unsigned j = 0; j < kop.qubits.size(); ++j) {
+    for (unsigned b = 0; b < nq; ++b)
+      if (qubits[b] == kop.qubits[j]) pos[j] = b;
+  }
+  const unsigned kd = 1u << kop.qubits.size();
+  unsigned mask = 0;
+  for (unsigned j = 0; j < kop.qubits.size(); ++j) mask |= 1u << pos[j];
+  auto sub = [&](unsigned x) {  // x's bits at the operator's qubits
+    unsigned v = 0;
+    for (unsigned j = 0; j < kop.qubits.size(); ++j) v |= ((x >> pos[j]) & 1u) << j;
+    return v;
+  };
+  std::vector<double> m(2 * size_t(D) * D, 0.0);
+  for (unsigned r = 0; r < D; ++r) {
+    for (unsigned c = 0; c < D; ++c) {
+      if ((r & ~mask) != (c & ~mask)) continue;
+      const unsigned rr = sub(r), cc = sub(c);
+      m[2 * (r * D + c)] = kop.kd_k[2 * (rr * kd + cc)];
+      m[2 * (r * D + c) + 1] = kop.kd_k[2 * (rr * kd + cc) + 1];
+    }
+  }
+  return m;
+}
+
+DeviceChannel MakeDeviceChannel(const Channel<double>& ch) {
+  DeviceChannel dc;
+  std::set<unsigned> qs;
+  for (const auto& kop : ch.kops) {
+    for (const auto& g : kop.ops) qs.insert(g.qubits.begin(), g.qubits.end());
+    if (!kop.unitary) qs.insert(kop.qubits.begin(), kop.qubits.end());
+  }
+  dc.qubits.assign(qs.begin(), qs.end());
+  if (dc.qubits.empty() || dc.qubits.size() > kMaxDeviceChannelQubits ||
+      ch.kops.empty() || ch.kops.size() > kMaxDeviceKraus) {
+    return dc;
+  }
+  for (const auto& kop : ch.kops) {
+    std::set<unsigned> kset;
+    for (const auto& g : kop.ops) kset.insert(g.qubits.begin(), g.qubits.end());
+    std::vector<unsigned> kq(kset.begin(), kset.end());
+    std::vector<float> k;
+    if (!kop.ops.empty()) {
+      // The operator's gates as one fused gate (lib/fuser.h:149-179).
+      FusedGate<double> fg{};
+      fg.kind = kop.ops[0].kind;
+      fg.time = kop.ops[0].time;
+      fg.qubits = kq;
+      fg.parent = &kop.ops[0];
+      for (const auto& g : kop.ops) fg.gates.push_back(&g);
+      CalculateFusedMatrix(fg);
+      k.assign(fg.matrix.begin(), fg.matrix.end());
+    }
+    std::vector<float> kdk;
+    if (!kop.unitary) {
+      if (kop.kd_k.empty() || kop.qubits.empty()) return dc;
+      const auto m = ExpandKdK(kop, dc.qubits);
+      kdk.assign(m.begin(), m.end());
+    }
+    dc.kq.push_back(std::move(kq));
+    dc.k.push_back(std::move(k));
+    dc.kdk.push_back(std::move(kdk));
+  }
+  dc.ok = true;
+  return dc;
+}
+
+void GridPairs(unsigned rows, unsigned cols, NoisyCircuit& nc) {
+  for (unsigned r = 0; r < rows; ++r) {
+    for (unsigned c = 0; c < cols; ++c) {
+      if (c + 1 < cols) {
+        nc.pairs.push_back(r * cols + c);
+        nc.pairs.push_back(r * cols + c + 1);
+      }
+      if (r + 1 < rows) {
+        nc.pairs.push_back(r * cols + c);
+        nc.pairs.push_back((r + 1) * cols + c);
+      }
+    }
+  }
+}
+
+void AddChannel(Channel<double>&& ch, NoisyCircuit& nc) {
+  nc.ops.push_back({1, nc.chans.size(), 0});
+  nc.dchans.push_back(MakeDeviceChannel(ch));
+  nc.chans.push_back(std::move(ch));
+}
 
 // One trajectory's plan.
 struct TrajPlan {
@@ -83,7 +203,6 @@ struct TrajBatchHost {
   std::vector<unsigned> order;      // device order -> caller index
   std::vector<uint32_t> pair_off;   // caller order
   std::vector<uint32_t> dec_off;    // caller order
-  std::vector<float> damp_k;        // K0 then K1 (2x2, FP32)
   void* handle = nullptr;
   uint32_t total_pairs = 0, total_decisions = 0;
   std::vector<double> zz;
@@ -106,37 +225,149 @@ bool Build(const std::string& text, unsigned rows, unsigned cols, double p,
     if (g == nullptr || g->qubits.empty() || g->qubits.size() > 4) {
       return false;
     }
+    // Deferred gates carry time 3t, so that the gates of a trajectory stay
+    // time-ordered per qubit with the depolarising Paulis (3t + 1) and the
+    // damping operators (3(2c + 1) + 2).
     const unsigned t = g->time;
-    nc.ops.push_back({0, i});
+    nc.ops.push_back({0, i, 3 * t});
     for (unsigned q : g->qubits) {
-      nc.ops.push_back({1, nc.chans.size()});
-      nc.chans.push_back(
-          Cirq::DepolarizingChannel<double>::Create(3 * t + 1, q, p));
+      AddChannel(Cirq::DepolarizingChannel<double>::Create(3 * t + 1, q, p),
+                 nc);
     }
     const bool cycle_end =
         i + 1 == ops.size() || OpTime(ops[i + 1]) / 2 != t / 2;
     if (cycle_end) {
       const unsigned c = t / 2;
       for (unsigned q = 0; q < n; ++q) {
-        nc.ops.push_back({1, nc.chans.size()});
-        nc.chans.push_back(Cirq::AmplitudeDampingChannel<double>::Create(
-            3 * (2 * c + 1) + 2, q, gamma));
+        AddChannel(Cirq::AmplitudeDampingChannel<double>::Create(
+                       3 * (2 * c + 1) + 2, q, gamma),
+                   nc);
       }
     }
   }
   nc.num_channels = nc.chans.size();
-  for (unsigned r = 0; r < rows; ++r) {
-    for (unsigned c = 0; c < cols; ++c) {
-      if (c + 1 < cols) {
-        nc.pairs.push_back(r * cols + c);
-        nc.pairs.push_back(r * cols + c + 1);
-      }
-      if (r + 1 < rows) {
-        nc.pairs.push_back(r * cols + c);
-        nc.pairs.push_back((r + 1) * cols + c);
+  GridPairs(rows, cols, nc);
+  return true;
+}
+
+// A channel line of noisy-circuit text, its first two fields (time and the
+// word "channel") already read:
+//   <name> <qubit> <params>   name: depolarize p | asymmetric_depolarize
+//       px py pz | amplitude_damp gamma | generalized_amplitude_damp p gamma
+//       | phase_damp gamma | phase_flip p | bit_flip p | reset  (qsim's
+//       factories, lib/channels_cirq.h)
+//   kraus <nq> <qubits, ascending> <number of operators>, then per
+//       operator: <unitary 0|1> <probability bound> <2 4^nq numbers: the
+//       matrix, row-major complex, bit j of its index <-> the j-th qubit>
+bool ParseChannel(unsigned time, unsigned n, std::istringstream& in,
+                  Channel<double>& ch) {
+  std::string name;
+  unsigned q = 0;
+  if (!(in >> name)) return false;
+  if (name == "kraus") {
+    unsigned nq = 0, nk = 0;
+    if (!(in >> nq) || nq < 1 || nq > 4) return false;
+    std::vector<unsigned> qs(nq);
+    for (unsigned j = 0; j < nq; ++j) {
+      if (!(in >> qs[j]) || qs[j] >= n || (j > 0 && qs[j] <= qs[j - 1])) {
+        return false;
       }
     }
+    if (!(in >> nk) || nk < 1 || nk > 64) return false;
+    ch = Channel<double>{{kChannel, time, qs}, {}};
+    const size_t len = size_t{2} << (2 * nq);
+    for (unsigned i = 0; i < nk; ++i) {
+      int unitary = 0;
+      double prob = 0;
+      Matrix<double> m(len);
+      if (!(in >> unitary >> prob)) return false;
+      for (auto& v : m) {
+        if (!(in >> v)) return false;
+      }
+      KrausOperator<double> kop;
+      kop.unitary = unitary != 0;
+      kop.prob = prob;
+      kop.ops.push_back(CreateGate<Gate<double>, Cirq::MatrixGate<double>>(
+          time, qs, std::move(m)));
+      kop.CalculateKdKMatrix();
+      ch.kops.push_back(std::move(kop));
+    }
+    return true;
   }
+  if (!(in >> q) || q >= n) return false;
+  double a = 0, b = 0, c = 0;
+  if (name == "depolarize" && (in >> a)) {
+    ch = Cirq::DepolarizingChannel<double>::Create(time, q, a);
+  } else if (name == "asymmetric_depolarize" && (in >> a >> b >> c)) {
+    ch = Cirq::AsymmetricDepolarizingChannel<double>::Create(time, q, a, b, c);
+  } else if (name == "amplitude_damp" && (in >> a)) {
+    ch = Cirq::AmplitudeDampingChannel<double>::Create(time, q, a);
+  } else if (name == "generalized_amplitude_damp" && (in >> a >> b)) {
+    ch = Cirq::GeneralizedAmplitudeDampingChannel<double>::Create(time, q, a,
+                                                                 b);
+  } else if (name == "phase_damp" && (in >> a)) {
+    ch = Cirq::PhaseDampingChannel<double>::Create(time, q, a);
+  } else if (name == "phase_flip" && (in >> a)) {
+    ch = Cirq::PhaseFlipChannel<double>::Create(time, q, a);
+  } else if (name == "bit_flip" && (in >> a)) {
+    ch = Cirq::BitFlipChannel<double>::Create(time, q, a);
+  } else if (name == "reset") {
+    ch = Cirq::ResetChannel<double>::Create(time, q);
+  } else {
+    return false;
+  }
+  return true;
+}
+
+// Noisy-circuit text: qsim's circuit format (lib/circuit_qsim_parser.h)
+// with channel lines ("<time> channel ...", ParseChannel) among the gate
+// lines, in the order the operations apply.  Gates keep their times.
+bool BuildNoisy(const std::string& text, unsigned rows, unsigned cols,
+                NoisyCircuit& nc) {
+  std::istringstream lines(text);
+  std::string line;
+  unsigned n = 0;
+  while (std::getline(lines, line)) {
+    if (line.find_first_not_of(" \t\r") == std::string::npos) continue;
+    std::istringstream in(line);
+    if (!(in >> n) || n == 0 || n > 34) return false;
+    break;
+  }
+  if (n == 0 || rows * cols != n) return false;
+  nc.num_qubits = n;
+  nc.circuit.num_qubits = n;
+  while (std::getline(lines, line)) {
+    const size_t first = line.find_first_not_of(" \t\r");
+    if (first == std::string::npos || line[first] == '#') continue;
+    std::istringstream in(line);
+    unsigned time = 0;
+    std::string word;
+    if (!(in >> time >> word)) return false;
+    if (word == "channel") {
+      Channel<double> ch;
+      if (!ParseChannel(time, n, in, ch)) return false;
+      std::string extra;
+      if (in >> extra) return false;
+      AddChannel(std::move(ch), nc);
+      continue;
+    }
+    // A gate line, through qsim's parser.
+    Circuit<Operation<double>> one;
+    std::stringstream gs;
+    gs << n << "\n" << line << "\n";
+    if (!CircuitQsimParser<IO>::FromStream(~0u, "circuit", gs, one) ||
+        one.ops.size() != 1) {
+      return false;
+    }
+    const Gate<double>* g = OpGetAlternative<Gate<double>>(one.ops[0]);
+    if (g == nullptr || g->qubits.empty() || g->qubits.size() > 4) {
+      return false;
+    }
+    nc.ops.push_back({0, nc.circuit.ops.size(), g->time});
+    nc.circuit.ops.push_back(std::move(one.ops[0]));
+  }
+  nc.num_channels = nc.chans.size();
+  GridPairs(rows, cols, nc);
   return true;
 }
 
@@ -282,8 +513,10 @@ uint64_t MatrixHash(const std::vector<float>& m) {
   return h;
 }
 
-// RunIteration's sampling (lib/qtrajectory.h:270-356) for one trajectory.
-void PlanTrajectory(const NoisyCircuit& nc, uint64_t rep,
+// RunIteration's sampling (lib/qtrajectory.h:270-370) for one trajectory;
+// false if a channel the bounds do not decide is beyond the device's
+// limits.
+bool PlanTrajectory(const NoisyCircuit& nc, uint64_t rep,
                     const FusionSpec& spec, FusedMatrixCache& cache,
                     TrajPlan& plan) {
   const unsigned n = nc.num_qubits;
@@ -299,7 +532,7 @@ void PlanTrajectory(const NoisyCircuit& nc, uint64_t rep,
     const double r = distr(rgen);
     if (op.kind == 0) {
       const auto& g = *OpGetAlternative<Gate<double>>(nc.circuit.ops[op.index]);
-      deferred.push_back(Timed(g, 3 * g.time));
+      deferred.push_back(Timed(g, op.time));
       continue;
     }
     const Channel<double>& ch = nc.chans[op.index];
@@ -318,19 +551,43 @@ void PlanTrajectory(const NoisyCircuit& nc, uint64_t rep,
       unitary = unitary && ch.kops[picked].unitary;
       continue;
     }
-    // The bounds do not decide: the state does (two-operator damping).  A
-    // non-unitary history is normalised first, as NormalizeState does.
+    // The bounds do not decide: the state does, on the device (the walk
+    // over the operators' <psi|K^dagger K|psi> of lib/qtrajectory.h:330-370,
+    // on the normalised state where the trajectory is not unitary).
+    const DeviceChannel& dc = nc.dchans[op.index];
+    if (!dc.ok) return false;
     Flush(deferred, n, spec, cache, plan);
-    qsim_hip_traj_op o{};
-    o.kind = QSIM_HIP_TRAJ_DAMP_DECIDE;
-    o.q[0] = ch.qubits[0];
-    o.r = float(r);
-    o.kdk1 = float(ch.kops[0].kd_k[6]);
-    o.slot = plan.num_decisions++;
-    o.pad = unitary ? 0 : 1;
-    plan.ops.push_back(o);
+    qsim_hip_traj_op h{};
+    h.kind = QSIM_HIP_TRAJ_CHANNEL;
+    h.k = ch.kops.size();
+    h.moff2 = dc.qubits.size();
+    for (unsigned b = 0; b < dc.qubits.size(); ++b) h.q[b] = dc.qubits[b];
+    h.r = r;
+    h.cp = cp;
+    h.slot = plan.num_decisions++;
+    h.pad = unitary ? 0 : 1;
+    plan.ops.push_back(h);
     plan.mats.emplace_back();  // no matrix of its own
-    plan.kraus.push_back(-2 - int32_t(o.slot));
+    for (size_t i = 0; i < ch.kops.size(); ++i) {
+      qsim_hip_traj_op ko{};
+      ko.kind = QSIM_HIP_TRAJ_KRAUS;
+      ko.k = dc.k[i].empty() ? 0 : dc.kq[i].size();
+      for (unsigned b = 0; b < ko.k; ++b) ko.q[b] = dc.kq[i][b];
+      ko.r = ch.kops[i].prob;
+      ko.pad = ch.kops[i].unitary ? 1 : 0;
+      // K then K^dagger K; their offsets are set when the batch is
+      // assembled (moff2 - moff = the size of K).
+      std::vector<float> m(dc.k[i]);
+      m.insert(m.end(), dc.kdk[i].begin(), dc.kdk[i].end());
+      ko.moff2 = dc.k[i].size();
+      plan.ops.push_back(ko);
+      plan.mats.push_back(std::move(m));
+    }
+    plan.kraus.push_back(-2 - int32_t(h.slot));
+    // A chosen operator makes the state non-unitary; where none is chosen
+    // (possible only if the last operator is unitary) the state was just
+    // normalised.  Either way the next decision normalising first is
+    // harmless, so the flag is cleared.
     unitary = false;
   }
   Flush(deferred, n, spec, cache, plan);
@@ -339,6 +596,7 @@ void PlanTrajectory(const NoisyCircuit& nc, uint64_t rep,
   for (size_t i = 0; i < plan.mats.size(); ++i) {
     plan.mhash[i] = MatrixHash(plan.mats[i]);
   }
+  return true;
 }
 
 }  // namespace
@@ -364,10 +622,27 @@ void* qsim_traj_create(const char* const* circuits, unsigned num_circuits,
       return nullptr;
     }
   }
-  // The two damping operators as the channel holds them, rounded to FP32.
-  auto ad = Cirq::AmplitudeDampingChannel<double>::Create(0, 0, gamma_damp);
-  for (int k = 0; k < 2; ++k)
-    for (double v : ad.kops[k].ops[0].matrix) b->damp_k.push_back(float(v));
+  return b.release();
+}
+
+void* qsim_traj_create_noisy(const char* const* texts, unsigned num_circuits,
+                             const unsigned* rows, const unsigned* cols) {
+  if (texts == nullptr || rows == nullptr || cols == nullptr ||
+      num_circuits == 0) {
+    return nullptr;
+  }
+  auto b = std::make_unique<TrajBatchHost>();
+  b->circuits.resize(num_circuits);
+  for (unsigned i = 0; i < num_circuits; ++i) {
+    try {
+      if (texts[i] == nullptr ||
+          !BuildNoisy(texts[i], rows[i], cols[i], b->circuits[i])) {
+        return nullptr;
+      }
+    } catch (const std::exception&) {  // qsim's parser and allocations
+      return nullptr;
+    }
+  }
   return b.release();
 }
 
@@ -399,14 +674,16 @@ int PlanBatch(TrajBatchHost* b, const unsigned* circuit_of,
   std::vector<FusedMatrixCache> caches(omp_get_max_threads());
   // An exception may not leave the parallel region: a failed host
   // allocation is caught per trajectory and returned as an error.
-  std::atomic<bool> out_of_memory{false};
+  std::atomic<bool> out_of_memory{false}, beyond_device{false};
 #pragma omp parallel for schedule(dynamic, 4)
   for (int t = 0; t < int(num_traj); ++t) {
     b->plans[t].circuit = circuit_of[t];
     b->plans[t].rep = reps[t];
     try {
-      PlanTrajectory(b->circuits[circuit_of[t]], reps[t], spec,
-                     caches[omp_get_thread_num()], b->plans[t]);
+      if (!PlanTrajectory(b->circuits[circuit_of[t]], reps[t], spec,
+                          caches[omp_get_thread_num()], b->plans[t])) {
+        beyond_device = true;
+      }
     } catch (const std::bad_alloc&) {
       out_of_memory = true;
     }
@@ -414,6 +691,12 @@ int PlanBatch(TrajBatchHost* b, const unsigned* circuit_of,
   if (out_of_memory) {
     b->plans.clear();
     return QSIM_HIP_ERR_RUNTIME;
+  }
+  if (beyond_device) {
+    // A channel on more than 2 qubits or with more than 16 operators whose
+    // draw its bounds did not decide.
+    b->plans.clear();
+    return QSIM_HIP_ERR_UNSUPPORTED;
   }
   // Device order: largest states first.
   b->order.resize(num_traj);
@@ -424,7 +707,7 @@ int PlanBatch(TrajBatchHost* b, const unsigned* circuit_of,
   });
   // Concatenate streams; identical matrices share one copy (found by their
   // hashes, computed in the parallel planning, and confirmed bit for bit).
-  std::vector<float> pool(b->damp_k);
+  std::vector<float> pool;
   std::unordered_map<uint64_t, std::vector<uint32_t>> seen;  // hash -> offsets
   std::vector<qsim_hip_traj_op> ops;
   std::vector<qsim_hip_traj_desc> descs;
@@ -471,10 +754,16 @@ int PlanBatch(TrajBatchHost* b, const unsigned* circuit_of,
         o.moff = off;
       } else if (o.kind == QSIM_HIP_TRAJ_TILE_PASS) {
         o.moff = 0;
-      } else {
-        o.moff = 0;
-        o.moff2 = 8;
+      } else if (o.kind == QSIM_HIP_TRAJ_CHANNEL) {
         o.slot += b->dec_off[t];
+      } else if (o.kind == QSIM_HIP_TRAJ_KRAUS) {
+        // K at moff, K^dagger K right after it (the plan's moff2 holds the
+        // size of K).
+        const auto& m = pl.mats[i];
+        const uint32_t ksize = o.moff2;
+        o.moff = uint32_t(pool.size());
+        o.moff2 = o.moff + ksize;
+        pool.insert(pool.end(), m.begin(), m.end());
       }
       ops.push_back(o);
     }

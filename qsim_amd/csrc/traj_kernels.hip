@@ -10,14 +10,17 @@
 //
 //   gate      a fused gate (k <= 4, FP32 matrix in LDS) over all 2^(n-k)
 //             groups of the trajectory's state;
-//   decision  an amplitude-damping channel whose draw fell past the
-//             probability bound: FP64 norm and |1>-population of the qubit,
-//             the choice between K0 and K1 exactly as RunIteration makes it
-//             (lib/qtrajectory.h:322-356), then the chosen operator applied.
+//   channel   a channel whose draw fell past the probability bounds: one
+//             FP64 sweep for the norm and every Kraus operator's
+//             <psi|K^dagger K|psi>, the choice exactly as RunIteration makes
+//             it (lib/qtrajectory.h:330-370), then the chosen operator
+//             applied (scaled to the normalised state where the trajectory
+//             is not unitary).
 //
 // The Kraus operators that the bounds decide (depolarising Paulis, damping
-// K0 when the draw is below 1 - gamma) are sampled on the host from the same
-// mt19937(rep) draws and folded into the fused gates.  At the end each
+// K0 when the draw is below 1 - gamma, any channel's operators whose bound
+// covers the draw) are sampled on the host from the same mt19937(rep)
+// draws and folded into the fused gates.  At the end each
 // trajectory writes <Z_a Z_b> of its qubit pairs, FP64-accumulated and
 // divided by the norm (the normalisation of lib/qtrajectory.h:377-383).
 //
@@ -128,77 +131,153 @@ __device__ void TrajGate(float2* __restrict__ s, unsigned n,
   __syncthreads();
 }
 
-// Amplitude-damping decision on qubit q: FP64 populations of |0> and |1>,
-// the choice of K0 or K1, the chosen operator applied.
+// Most Kraus operators of a channel decided on the device, and its most
+// qubits.
+constexpr unsigned kMaxKraus = 16;
+constexpr unsigned kMaxChannelQubits = 2;
+
+// Scales the state by a (a sweep).
 template <int NT>
-__device__ void TrajDecide(float2* __restrict__ s, unsigned n,
-                           const qsim_hip_traj_op& o,
-                           const float* __restrict__ mats,
-                           int32_t* __restrict__ dec_out, double* red) {
+__device__ void ScaleState(float2* __restrict__ s, uint64_t N, float a) {
+  for (uint64_t i = threadIdx.x; i < N; i += NT) {
+    const float2 v = s[i];
+    s[i] = float2{v.x * a, v.y * a};
+  }
+}
+
+// A channel that the draw's bounds did not decide (header ops[oi], its
+// Kraus operators at ops[oi + 1 ..]), decided as RunIteration decides it
+// (lib/qtrajectory.h:330-370) and in the same order: the non-unitary
+// operators one at a time, each with a sweep for <psi|K^dagger K|psi>
+// (each group's value in FP32, summed in FP64) -- the first sweep also
+// giving the norm where the trajectory is not unitary, so that the
+// probabilities are those of the normalised state (NormalizeState,
+// lib/qtrajectory.h:449-456) -- until the draw falls below the running
+// sum; the chosen operator is then applied, scaled by the normalisation.
+// lm: LDS of 2 x 4^NQ floats (at least one 4 x 4 complex matrix); vals:
+// LDS of 2 doubles.
+template <int NT, int NQ>
+__device__ void TrajChannelQ(float2* __restrict__ s, unsigned n,
+                             const qsim_hip_traj_op* __restrict__ ops,
+                             uint64_t oi, const float* __restrict__ mats,
+                             int32_t* __restrict__ dec_out, double* red,
+                             float* lm, double* vals) {
+  constexpr unsigned D = 1u << NQ;
+  const qsim_hip_traj_op h = ops[oi];
+  const unsigned M = h.k;
   const uint64_t N = uint64_t{1} << n;
-  const unsigned q = o.q[0];
-  constexpr unsigned kS = kSweep<NT>;
-  double p0 = 0.0, p1 = 0.0;
-  for (uint64_t i0 = threadIdx.x; i0 < N; i0 += kS * NT) {
-    float2 a[kS];
+  const uint64_t ngroups = N >> NQ;
+  const unsigned q0 = h.q[0], q1 = NQ > 1 ? h.q[1] : 0;
+  const bool normalise = h.pad & 1u;
+  bool have_norm = !normalise;
+  double norm = 1.0;
+  double cp = h.cp, max_prob = 0.0;
+  int max_index = 0, pick = -1;
+  for (unsigned i = 0; i < M; ++i) {
+    const qsim_hip_traj_op c = ops[oi + 1 + i];
+    if (c.pad & 1u) continue;  // unitary: decided by its bound alone
+    __syncthreads();           // earlier reads of lm are done
+    for (unsigned e = threadIdx.x; e < 2 * D * D; e += NT)
+      lm[e] = mats[c.moff2 + e];
+    __syncthreads();
+    double acc = 0.0, nacc = 0.0;
+    for (uint64_t g = threadIdx.x; g < ngroups; g += NT) {
+      uint64_t base =
+          ((g >> q0) << (q0 + 1)) | (g & ((uint64_t{1} << q0) - 1));
+      if (NQ > 1) {
+        base = ((base >> q1) << (q1 + 1)) | (base & ((uint64_t{1} << q1) - 1));
+      }
+      float2 v[D];
 #pragma unroll
-    for (unsigned u = 0; u < kS; ++u) {
-      const uint64_t i = i0 + u * NT;
-      a[u] = i < N ? s[i] : float2{0.0f, 0.0f};
-    }
+      for (unsigned j = 0; j < D; ++j) {
+        const uint64_t off = ((j & 1u) ? uint64_t{1} << q0 : 0) |
+                             ((j & 2u) ? uint64_t{1} << q1 : 0);
+        v[j] = s[base + off];
+      }
+      float w = 0.0f;  // Re v^dagger (K^dagger K) v
 #pragma unroll
-    for (unsigned u = 0; u < kS; ++u) {
-      const double w = double(a[u].x) * a[u].x + double(a[u].y) * a[u].y;
-      if (((i0 + u * NT) >> q) & 1) p1 += w; else p0 += w;
-    }
-  }
-  p0 = BlockSum<NT>(p0, red);
-  p1 = BlockSum<NT>(p1, red);
-  if (threadIdx.x == 0) {
-    red[NT / 64] = p0;
-    red[NT / 64 + 1] = p1;
-  }
-  __syncthreads();
-  p0 = red[NT / 64];
-  p1 = red[NT / 64 + 1];
-  __syncthreads();
-  // RunIteration normalises a non-unitary state first, then samples K0 iff
-  // r < <psi|K0^dagger K0|psi> (the bound 1 - gamma cancels out of cp).
-  const double norm = p0 + p1;
-  const double prob0 = (p0 + double(o.kdk1) * p1) / norm;
-  const int pick = double(o.r) < prob0 ? 0 : 1;
-  const float scale = o.pad & 1u ? float(1.0 / std::sqrt(norm)) : 1.0f;
-  const float* km = mats + (pick == 0 ? o.moff : o.moff2);
-  const float m00r = km[0] * scale, m00i = km[1] * scale;
-  const float m01r = km[2] * scale, m01i = km[3] * scale;
-  const float m10r = km[4] * scale, m10i = km[5] * scale;
-  const float m11r = km[6] * scale, m11i = km[7] * scale;
-  const uint64_t half = N >> 1, qb = uint64_t{1} << q;
-  constexpr unsigned kPairs = kS / 2;
-  for (uint64_t g0 = threadIdx.x; g0 < half; g0 += kPairs * NT) {
-    float2 a[kPairs], b[kPairs];
-    uint64_t i0[kPairs];
+      for (unsigned r = 0; r < D; ++r) {
+        float re = 0.0f, im = 0.0f;
 #pragma unroll
-    for (unsigned u = 0; u < kPairs; ++u) {
-      const uint64_t g = g0 + u * NT;
-      i0[u] = ((g >> q) << (q + 1)) | (g & (qb - 1));
-      if (g < half) {
-        a[u] = s[i0[u]];
-        b[u] = s[i0[u] + qb];
+        for (unsigned k = 0; k < D; ++k) {
+          const float mr = lm[2 * (r * D + k)], mi = lm[2 * (r * D + k) + 1];
+          re += mr * v[k].x - mi * v[k].y;
+          im += mr * v[k].y + mi * v[k].x;
+        }
+        w += v[r].x * re + v[r].y * im;
+      }
+      acc += double(w);
+      if (!have_norm) {
+#pragma unroll
+        for (unsigned j = 0; j < D; ++j)
+          nacc += double(v[j].x) * v[j].x + double(v[j].y) * v[j].y;
       }
     }
-#pragma unroll
-    for (unsigned u = 0; u < kPairs; ++u) {
-      if (g0 + u * NT < half) {
-        s[i0[u]] = float2{m00r * a[u].x - m00i * a[u].y + m01r * b[u].x - m01i * b[u].y,
-                          m00r * a[u].y + m00i * a[u].x + m01r * b[u].y + m01i * b[u].x};
-        s[i0[u] + qb] = float2{m10r * a[u].x - m10i * a[u].y + m11r * b[u].x - m11i * b[u].y,
-                               m10r * a[u].y + m10i * a[u].x + m11r * b[u].y + m11i * b[u].x};
-      }
+    acc = BlockSum<NT>(acc, red);
+    if (threadIdx.x == 0) vals[0] = acc;
+    if (!have_norm) {
+      nacc = BlockSum<NT>(nacc, red);
+      if (threadIdx.x == 0) vals[1] = nacc;
+    }
+    __syncthreads();
+    if (!have_norm) {
+      norm = vals[1];
+      have_norm = true;
+    }
+    // The walk, the same in every thread.
+    const double prob = vals[0] / norm;
+    if (prob > max_prob) {
+      max_prob = prob;
+      max_index = int(i);
+    }
+    cp += prob - c.r;
+    if (h.r < cp || i == M - 1) {
+      pick = h.r < cp ? int(i) : max_index;
+      break;
     }
   }
-  if (threadIdx.x == 0) dec_out[o.slot] = pick;
+  if (!have_norm) {
+    // Every operator unitary: nothing chosen, the state only normalised.
+    double nacc = 0.0;
+    for (uint64_t i = threadIdx.x; i < N; i += NT) {
+      const float2 v = s[i];
+      nacc += double(v.x) * v.x + double(v.y) * v.y;
+    }
+    nacc = BlockSum<NT>(nacc, red);
+    if (threadIdx.x == 0) vals[1] = nacc;
+    __syncthreads();
+    norm = vals[1];
+  }
+  if (threadIdx.x == 0) dec_out[h.slot] = pick;
+  __syncthreads();  // every thread is done with lm and vals
+  const float a = normalise ? float(1.0 / std::sqrt(norm)) : 1.0f;
+  const qsim_hip_traj_op c = ops[oi + 1 + (pick >= 0 ? pick : 0)];
+  if (pick < 0 || c.k == 0) {
+    // Nothing applied (or the identity): the state is only normalised.
+    if (normalise) ScaleState<NT>(s, N, a);
+    __syncthreads();
+    return;
+  }
+  const unsigned DK = 1u << c.k;
+  for (unsigned e = threadIdx.x; e < 2 * DK * DK; e += NT)
+    lm[e] = mats[c.moff + e] * a;
   __syncthreads();
+  if (c.k == 1) ApplyGateK<NT, 1>(s, n, c, lm);
+  else ApplyGateK<NT, 2>(s, n, c, lm);
+  __syncthreads();
+}
+
+template <int NT>
+__device__ void TrajChannel(float2* __restrict__ s, unsigned n,
+                            const qsim_hip_traj_op* __restrict__ ops,
+                            uint64_t oi, const float* __restrict__ mats,
+                            int32_t* __restrict__ dec_out, double* red,
+                            float* lm, double* vals) {
+  static_assert(kMaxChannelQubits == 2, "channel sizes");
+  if (ops[oi].moff2 == 1)
+    TrajChannelQ<NT, 1>(s, n, ops, oi, mats, dec_out, red, lm, vals);
+  else
+    TrajChannelQ<NT, 2>(s, n, ops, oi, mats, dec_out, red, lm, vals);
 }
 
 // Signs (-1)^(bit a + bit b) of index i for the pairs (pa[j], pb[j]),
@@ -309,6 +388,7 @@ TrajectoryKernel(const qsim_hip_traj_desc* __restrict__ descs,
                  int32_t* __restrict__ dec_out) {
   __shared__ float lm[2 * 16 * 16];
   __shared__ double red[kRedWords<NT>];
+  __shared__ double vals[2];
   const qsim_hip_traj_desc d = descs[blockIdx.x];
   const unsigned n = d.num_qubits;
   float2* s = pool + d.state_offset;
@@ -318,8 +398,9 @@ TrajectoryKernel(const qsim_hip_traj_desc* __restrict__ descs,
     const qsim_hip_traj_op o = ops[oi];
     if (o.kind == QSIM_HIP_TRAJ_GATE) {
       TrajGate<NT>(s, n, o, mats, lm);
-    } else if (o.kind == QSIM_HIP_TRAJ_DAMP_DECIDE) {
-      TrajDecide<NT>(s, n, o, mats, dec_out, red);
+    } else if (o.kind == QSIM_HIP_TRAJ_CHANNEL) {
+      TrajChannel<NT>(s, n, ops, oi, mats, dec_out, red, lm, vals);
+      oi += o.k;  // its Kraus operators
     }
   }
   TrajFinish<NT>(s, d, pairs, zz_out, norm_out, red);
@@ -470,9 +551,15 @@ TrajTileKernel(const qsim_hip_traj_desc* __restrict__ descs,
                int32_t* __restrict__ dec_out) {
   constexpr int NT = BlockShape<TB>::NT;
   extern __shared__ char lds_raw[];
+  // The ring area holds the channel and gate scratch between tile passes.
   char* aux = lds_raw + BlockShape<TB>::kTileBytes;
+  constexpr size_t kLmBytes = 2 * 16 * 16 * sizeof(float);
+  static_assert(kLmBytes + (kRedWords<NT> + 2) * sizeof(double) <=
+                    2 * kOpWords * sizeof(uint32_t),
+                "scratch fits the ring area");
   float* lm = reinterpret_cast<float*>(aux);
-  double* red = reinterpret_cast<double*>(aux + 2 * 16 * 16 * sizeof(float));
+  double* red = reinterpret_cast<double*>(aux + kLmBytes);
+  double* vals = red + kRedWords<NT>;
   const qsim_hip_traj_desc d = descs[blockIdx.x];
   const unsigned n = d.num_qubits;
   float2* s = pool + d.state_offset;
@@ -489,8 +576,9 @@ TrajTileKernel(const qsim_hip_traj_desc* __restrict__ descs,
         TrajTileRun<TB, false>(s, passes + o.moff, recs, lds_raw);
       }
       oi += o.k;  // the pass's gates
-    } else {
-      TrajDecide<NT>(s, n, o, mats, dec_out, red);
+    } else if (o.kind == QSIM_HIP_TRAJ_CHANNEL) {
+      TrajChannel<NT>(s, n, ops, oi, mats, dec_out, red, lm, vals);
+      oi += o.k;  // its Kraus operators
     }
   }
   TrajFinish<NT>(s, d, pairs, zz_out, norm_out, red);
@@ -784,9 +872,30 @@ int qsim_hip_traj_prepare(const qsim_hip_traj_desc* trajs, unsigned num_traj,
              uint64_t(o.moff) + (size_t{2} << (2 * o.k)) <= num_floats;
         for (unsigned b = 0; ok && b < o.k; ++b)
           ok = o.q[b] < n && (b == 0 || o.q[b] > o.q[b - 1]);
-      } else if (o.kind == QSIM_HIP_TRAJ_DAMP_DECIDE) {
-        ok = o.q[0] < n && uint64_t(o.moff) + 8 <= num_floats &&
-             uint64_t(o.moff2) + 8 <= num_floats && o.slot < num_decisions;
+      } else if (o.kind == QSIM_HIP_TRAJ_CHANNEL) {
+        // The header of a channel decision and its Kraus operators: each K
+        // on qubits of the channel, K^dagger K over the channel's qubits.
+        const unsigned nq = o.moff2;
+        ok = o.k >= 1 && o.k <= kMaxKraus && i + o.k < d[t].op_end &&
+             nq >= 1 && nq <= kMaxChannelQubits && o.slot < num_decisions;
+        uint64_t cmask = 0;
+        for (unsigned b = 0; ok && b < nq; ++b) {
+          ok = o.q[b] < n && (b == 0 || o.q[b] > o.q[b - 1]);
+          if (ok) cmask |= uint64_t{1} << o.q[b];
+        }
+        for (unsigned j = 1; ok && j <= o.k; ++j) {
+          const qsim_hip_traj_op& c = ops[i + j];
+          ok = c.kind == QSIM_HIP_TRAJ_KRAUS && c.k <= nq &&
+               (c.k == 0 ||
+                uint64_t(c.moff) + (size_t{2} << (2 * c.k)) <= num_floats) &&
+               ((c.pad & 1u) ||
+                uint64_t(c.moff2) + (size_t{2} << (2 * nq)) <= num_floats);
+          for (unsigned b = 0; ok && b < c.k; ++b) {
+            ok = c.q[b] < 64 && ((cmask >> c.q[b]) & 1u) &&
+                 (b == 0 || c.q[b] > c.q[b - 1]);
+          }
+        }
+        if (ok) i += o.k;  // the Kraus operators are part of the header
       } else if (o.kind == QSIM_HIP_TRAJ_TILE_PASS) {
         // The header of a pass over the gates that follow it (each checked
         // as a gate in its turn; their tile membership when the pass is

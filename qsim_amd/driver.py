@@ -62,6 +62,7 @@ def driver_lib():
                 ("qsim_driver_set_cost_model", None, [vp]),
                 ("qsim_traj_create", vp, [vp, u, up, up, ctypes.c_double,
                                           ctypes.c_double]),
+                ("qsim_traj_create_noisy", vp, [vp, u, up, up]),
                 ("qsim_traj_num_channels", u, [vp, u]),
                 ("qsim_traj_plan", i, [vp, up, vp, u, u]),
                 ("qsim_traj_plan_tiled", i, [vp, up, vp, u, u, u]),

@@ -1,9 +1,11 @@
 """Batched noisy trajectories on the GPU (include/qsim_hip_driver.h,
 qsim_traj_*): many trajectories of uneven qubit counts in one launch.
 
-The noisy circuit is a qsim grid circuit with a depolarising channel after
-every gate (on each of its qubits) and an amplitude-damping channel on every
-qubit after each cycle.  Trajectory sampling follows qsim's RunIteration
+A noisy circuit is either a qsim grid circuit with a depolarising channel
+after every gate (on each of its qubits) and an amplitude-damping channel
+on every qubit after each cycle (TrajectoryBatch), or any noisy-circuit text
+of qsim gates and channels (TrajectoryBatch.from_noisy; the format is
+qsim_traj_create_noisy's).  Trajectory sampling follows qsim's RunIteration
 (lib/qtrajectory.h:240-387) with mt19937(rep) draws, so the Kraus choices
 equal the CPU trajectory simulator's for the same repetition ids.
 """
@@ -57,6 +59,30 @@ class TrajectoryBatch:
         self.p_depol, self.gamma_damp = p_depol, gamma_damp
         self.circuit_of = None
         self.reps = None
+
+    @classmethod
+    def from_noisy(cls, texts, shapes):
+        """A batch over noisy-circuit texts (qsim_traj_create_noisy), each
+        on a rows x cols grid (``shapes``: (rows, cols) per text) whose
+        neighbour pairs get <Z_a Z_b>."""
+        self = cls.__new__(cls)
+        enc = [t.encode() for t in texts]
+        arr = (ctypes.c_char_p * len(enc))(*enc)
+        r = np.asarray([s[0] for s in shapes], np.uint32)
+        c = np.asarray([s[1] for s in shapes], np.uint32)
+        self.h = driver_lib().qsim_traj_create_noisy(
+            ctypes.cast(arr, ctypes.c_void_p), len(enc),
+            r.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+            c.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+        if not self.h:
+            raise ValueError("cannot parse the noisy circuits")
+        self.sizes = [int(a) * int(b) for a, b in shapes]
+        self.texts = list(texts)
+        self.cycles = self.seed = None
+        self.p_depol = self.gamma_damp = None
+        self.circuit_of = None
+        self.reps = None
+        return self
 
     def num_channels(self, circuit: int) -> int:
         return driver_lib().qsim_traj_num_channels(self.h, circuit)

@@ -3,12 +3,15 @@ against qsim's own QuantumTrajectorySimulator on the CPU (oracle/_ref,
 SimulatorAVX) with the same repetition ids.
 
 Bars: the Kraus index of every channel identical to the CPU run (the host
-replays the same mt19937 draws; the state-dependent damping decisions are
+replays the same mt19937 draws; the state-dependent channel decisions are
 taken on the device); <Z_a Z_b> on grid neighbours within 2e-5 of the CPU
 values (FP32 states on both sides, FP64 sums); graph replay bit-identical to
 the plain launch.  The tiled kernel (tile passes: runs of gates applied to
 LDS-staged tiles in split-BF16 MFMA arithmetic, one stream of the state per
-pass) is held to the same bars.
+pass) is held to the same bars.  Noisy circuits come from the fixed
+depolarise + damp rule and from noisy-circuit text with every channel kind
+qsim builds (single-qubit channels, reset, 2-qubit Kraus sets); the full
+1024-trajectory batch is checked on a sample of its trajectories.
 """
 
 import ctypes
@@ -183,10 +186,174 @@ def test_replanning_reuses_the_batch():
     b.free()
 
 
+def _fmt_matrix(m):
+    return " ".join(f"{x:.17g}" for v in np.asarray(m).reshape(-1)
+                    for x in (v.real, v.imag))
+
+
+def _damp(g):
+    a0 = np.array([[1, 0], [0, np.sqrt(1 - g)]], complex)
+    a1 = np.array([[0, np.sqrt(g)], [0, 0]], complex)
+    return a0, a1
+
+
+def _kron2(a, b):
+    # Bit 0 of the index <-> the first (lower) qubit: K[r0 + 2 r1, c0 + 2 c1]
+    # = a[r0, c0] b[r1, c1].
+    return np.kron(b, a)
+
+
+SINGLE = ["amplitude_damp 0.08", "phase_damp 0.1",
+          "generalized_amplitude_damp 0.3 0.1", "bit_flip 0.05",
+          "asymmetric_depolarize 0.02 0.03 0.04", "depolarize 0.05",
+          "phase_flip 0.05", "amplitude_damp 0.2"]
+
+
+def noisy_text(n, cycles, seed=0):
+    """A noisy grid circuit as text (qsim_traj_create_noisy's format), the
+    noise placed as qsim's MakeNoisy places it (lib/circuit_noisy.h:33-63):
+    every gate is followed by a channel on each of its qubits, cycling
+    through qsim's single-qubit channels (a reset on every 29th); after
+    every cycle two 2-qubit Kraus channels on neighbouring qubits --
+    correlated amplitude damping (four non-unitary operators) and a
+    mixed-unitary II / ZZ / XX channel.  Times keep every qubit's operations
+    in strict time order, as qsim's fuser requires of deferred gates
+    (lib/fuser_mqubit.h:1220-1224): gate 4t, its channels 4t + 1, the
+    cycle's two Kraus channels 4t + 2 and 4t + 3."""
+    rows, cols = circuits.grid_shape(n)
+    _, ops = circuits.grid_circuit_ops(rows, cols, cycles, seed)
+    a0, a1 = _damp(0.1)
+    damp2 = [(0, (1 - 0.1) ** 2, _kron2(a0, a0)), (0, 0.0, _kron2(a0, a1)),
+             (0, 0.0, _kron2(a1, a0)), (0, 0.0, _kron2(a1, a1))]
+    z, x, i2 = np.diag([1, -1]).astype(complex), np.array([[0, 1], [1, 0]],
+                                                          complex), np.eye(2)
+    mixed = [(1, 0.9, _kron2(i2, i2)), (1, 0.05, _kron2(z, z)),
+             (1, 0.05, _kron2(x, x))]
+    lines = [str(n)]
+    count = 0
+    last_t = None
+    for t, name, qs, prm in ops:
+        if last_t is not None and t // 2 != last_t // 2:
+            c = last_t // 2
+            a, b = (2 * c) % (n - 1), (2 * c) % (n - 1) + 1
+            for dt, kops in ((2, damp2), (3, mixed)):
+                body = " ".join(f"{u} {p!r} {_fmt_matrix(k)}"
+                                for u, p, k in kops)
+                lines.append(f"{4 * last_t + dt} channel kraus 2 {a} {b} "
+                             f"{len(kops)} {body}")
+        last_t = t
+        lines.append(" ".join([str(4 * t), name] + [str(q) for q in qs] +
+                              [repr(v) for v in prm]))
+        for q in qs:
+            count += 1
+            kind = "reset" if count % 29 == 0 else SINGLE[count % len(SINGLE)]
+            nm, *params = kind.split()
+            lines.append(" ".join([str(4 * t + 1), "channel", nm, str(q)] +
+                                  params))
+    return "\n".join(lines) + "\n"
+
+
+def check_noisy(tmp_path, sizes, cycles, reps_per_size, threads, kmax):
+    texts = [noisy_text(n, cycles, seed=3) for n in sizes]
+    shapes = [circuits.grid_shape(n) for n in sizes]
+    b = trajectories.TrajectoryBatch.from_noisy(texts, shapes)
+    circuit_of, reps = [], []
+    for ci in range(len(sizes)):
+        for r in range(reps_per_size):
+            circuit_of.append(ci)
+            reps.append(500 * ci + r)
+    b.plan(circuit_of, reps, kmax)
+    b.run(threads)
+    worst, nontrivial = 0.0, 0
+    for ci, (n, (rows, cols)) in enumerate(zip(sizes, shapes)):
+        f = tmp_path / f"noisy{n}.txt"
+        f.write_text(texts[ci])
+        lo = 500 * ci
+        runs, _ = oracle.run_reference_trajectories(
+            str(f), rows, cols, lo, lo + reps_per_size, max_fused=4,
+            threads=4, noisy_text=True)
+        ref = {r["rep"]: r for r in runs}
+        for t in range(len(reps)):
+            if circuit_of[t] != ci:
+                continue
+            k = b.kraus(t)
+            assert list(k) == ref[reps[t]]["kraus"], (n, reps[t])
+            nontrivial += int(np.sum(k > 0))
+            zz = b.zz(t)
+            worst = max(worst, float(np.max(np.abs(
+                zz - np.asarray(ref[reps[t]]["zz"])))))
+    print(f"noisy {sizes}: {len(reps)} trajectories, {nontrivial} "
+          f"non-trivial Kraus picks, max |dZZ| {worst:.2e}", flush=True)
+    assert worst < 2e-5
+    b.free()
+
+
+@pytest.mark.parametrize("threads,kmax", [(512, 4), (TILED, ("tile", 10, 4))])
+def test_any_channel_matches_qtrajectory(tmp_path, threads, kmax):
+    # qsim's single-qubit channels, reset and 2-qubit Kraus sets (non-unitary
+    # and mixed-unitary): identical Kraus choices (many of them decided on
+    # the device from <psi|K^dagger K|psi>) and <Z_a Z_b> within 2e-5.
+    check_noisy(tmp_path, [10, 12], 6, 12, threads, kmax)
+
+
+def test_channels_beyond_the_device_are_refused():
+    # A 3-qubit Kraus channel whose bounds do not decide every draw.
+    n = 9
+    k = np.eye(8, dtype=complex) * np.sqrt(0.5)
+    body = " ".join(f"0 0 {_fmt_matrix(k)}" for _ in range(2))
+    text = (f"{n}\n0 h 0\n1 channel kraus 3 0 1 2 2 {body}\n")
+    b = trajectories.TrajectoryBatch.from_noisy([text], [(3, 3)])
+    with pytest.raises(qsim_amd.QsimHipError) as e:
+        b.plan([0], [7], 4)
+    assert e.value.code == qsim_amd.ERR_UNSUPPORTED
+    b.free()
+
+
+def test_full_1024_batch_samples_match_qtrajectory(tmp_path):
+    # The benchmark's batch as it runs there (1024 trajectories of 18, 20
+    # and 22 qubits, 12 cycles, tiled 14/4 passes, one graph-replayed
+    # launch); a sample of its trajectories against qsim's CPU runs of the
+    # same repetitions.
+    sizes = [18, 20, 22]
+    b = trajectories.TrajectoryBatch(sizes, 12, p_depol=1e-3, gamma_damp=5e-3)
+    ntraj = 1024
+    circuit_of = [i % 3 for i in range(ntraj)]
+    reps = list(range(ntraj))
+    b.plan(circuit_of, reps, ("tile", 14, 4))
+    b.run(TILED, use_graph=True)
+    b.run(TILED, use_graph=True)
+    sample = sorted(set([0, 1, 2, 3, 4, 5, 340, 341, 342, 511, 512, 513,
+                         700, 701, 702, 1021, 1022, 1023] +
+                        list(range(100, 1000, 97))))
+    worst, checked = 0.0, 0
+    for ci, n in enumerate(sizes):
+        mine = [t for t in sample if circuit_of[t] == ci]
+        rows, cols = circuits.grid_shape(n)
+        f = tmp_path / f"g{n}.txt"
+        f.write_text(circuits.grid_circuit(n, 12, 0))
+        ref = {}
+        for t in mine:
+            runs, _ = oracle.run_reference_trajectories(
+                str(f), rows, cols, t, t + 1, 1e-3, 5e-3, max_fused=4,
+                threads=8)
+            ref[t] = runs[0]
+        for t in mine:
+            assert list(b.kraus(t)) == ref[t]["kraus"], (n, t)
+            worst = max(worst, float(np.max(np.abs(
+                b.zz(t) - np.asarray(ref[t]["zz"])))))
+            checked += 1
+    print(f"1024-trajectory batch: {checked} sampled trajectories against "
+          f"qsim's QuantumTrajectorySimulator, max |dZZ| {worst:.2e}",
+          flush=True)
+    assert worst < 2e-5
+    b.free()
+
+
 # The C-ABI structs (include/qsim_hip.h).
 OP = np.dtype([("kind", "<u4"), ("k", "<u4"), ("q", "<u4", (4,)),
-               ("moff", "<u4"), ("moff2", "<u4"), ("r", "<f4"),
-               ("kdk1", "<f4"), ("slot", "<u4"), ("pad", "<u4")])
+               ("moff", "<u4"), ("moff2", "<u4"), ("r", "<f8"),
+               ("cp", "<f8"), ("slot", "<u4"), ("pad", "<u4")])
+assert OP.itemsize == 56
 DESC = np.dtype([("num_qubits", "<u4"), ("num_pairs", "<u4"),
                  ("state_offset", "<u8"), ("op_begin", "<u8"),
                  ("op_end", "<u8"), ("pair_offset", "<u4"), ("pad", "<u4")])
@@ -266,3 +433,74 @@ def test_malformed_tile_passes_are_rejected(bad):
     rc, h = raw_batch(n, ops, mats)
     assert rc == qsim_amd.ERR_ARG, bad
     assert not h.value
+
+
+def channel_stream(n, nq=1, kraus=2):
+    """A channel decision (QSIM_HIP_TRAJ_CHANNEL) on qubits 0.. nq - 1 with
+    `kraus` amplitude-damping-like operators (QSIM_HIP_TRAJ_KRAUS)."""
+    ops = np.zeros(1 + kraus, OP)
+    ops["kind"][0] = 1
+    ops["k"][0] = kraus
+    ops["moff2"][0] = nq
+    ops["q"][0, :nq] = np.arange(nq)
+    ops["r"][0] = 0.999
+    ops["cp"][0] = 0.9
+    dim = 1 << nq
+    mats = []
+    for i in range(kraus):
+        ops["kind"][1 + i] = 3
+        ops["k"][1 + i] = 1
+        ops["q"][1 + i, 0] = 0
+        ops["moff"][1 + i] = sum(len(m) for m in mats)
+        mats.append(to_interleaved(np.eye(2) * 0.5, np.float32))
+        ops["moff2"][1 + i] = sum(len(m) for m in mats)
+        mats.append(to_interleaved(np.eye(dim) * 0.25, np.float32))
+        ops["r"][1 + i] = 0.45
+    return ops, np.concatenate(mats).astype(np.float32)
+
+
+def raw_channel_batch(n, ops, mats, num_decisions):
+    d = np.zeros(1, DESC)
+    d[0]["num_qubits"] = n
+    d[0]["op_end"] = len(ops)
+    h = ctypes.c_void_p()
+    pairs = np.zeros(2, np.uint32)
+    rc = qsim_amd.lib().qsim_hip_traj_prepare(
+        d.ctypes.data, 1, ops.ctypes.data, len(ops), mats.ctypes.data,
+        len(mats), pairs.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), 0,
+        num_decisions, ctypes.byref(h))
+    return rc, h
+
+
+@pytest.mark.parametrize("bad", ["ok", "k0", "past_end", "nq3", "slot",
+                                 "kraus_alone", "outside", "matrix_end",
+                                 "kraus_too_big"])
+def test_malformed_channel_streams_are_rejected(bad):
+    n = 8
+    ops, mats = channel_stream(n)
+    ndec = 1
+    if bad == "k0":
+        ops["k"][0] = 0
+    elif bad == "past_end":
+        ops["k"][0] = 3
+    elif bad == "nq3":
+        ops["moff2"][0] = 3
+        ops["q"][0, :3] = [0, 1, 2]
+    elif bad == "slot":
+        ndec = 0
+    elif bad == "kraus_alone":
+        ops = ops[1:].copy()
+    elif bad == "outside":
+        ops["q"][2, 0] = 5
+    elif bad == "matrix_end":
+        ops["moff2"][2] = len(mats) - 2
+    elif bad == "kraus_too_big":
+        ops["k"][1] = 2
+        ops["q"][1, :2] = [0, 1]
+    rc, h = raw_channel_batch(n, ops, mats, ndec)
+    if bad == "ok":
+        assert rc == qsim_amd.OK
+        qsim_amd.lib().qsim_hip_traj_free(h)
+    else:
+        assert rc == qsim_amd.ERR_ARG, bad
+        assert not h.value
