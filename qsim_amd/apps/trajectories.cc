@@ -611,18 +611,22 @@ void* qsim_traj_create(const char* const* circuits, unsigned num_circuits,
       gamma_damp > 1) {
     return nullptr;
   }
-  auto b = std::make_unique<TrajBatchHost>();
-  b->p = p_depol;
-  b->gamma = gamma_damp;
-  b->circuits.resize(num_circuits);
-  for (unsigned i = 0; i < num_circuits; ++i) {
-    if (circuits[i] == nullptr ||
-        !Build(circuits[i], rows[i], cols[i], p_depol, gamma_damp,
-               b->circuits[i])) {
-      return nullptr;
+  try {
+    auto b = std::make_unique<TrajBatchHost>();
+    b->p = p_depol;
+    b->gamma = gamma_damp;
+    b->circuits.resize(num_circuits);
+    for (unsigned i = 0; i < num_circuits; ++i) {
+      if (circuits[i] == nullptr ||
+          !Build(circuits[i], rows[i], cols[i], p_depol, gamma_damp,
+                 b->circuits[i])) {
+        return nullptr;
+      }
     }
+    return b.release();
+  } catch (const std::exception&) {  // qsim's parser and allocations
+    return nullptr;
   }
-  return b.release();
 }
 
 void* qsim_traj_create_noisy(const char* const* texts, unsigned num_circuits,
@@ -631,19 +635,19 @@ void* qsim_traj_create_noisy(const char* const* texts, unsigned num_circuits,
       num_circuits == 0) {
     return nullptr;
   }
-  auto b = std::make_unique<TrajBatchHost>();
-  b->circuits.resize(num_circuits);
-  for (unsigned i = 0; i < num_circuits; ++i) {
-    try {
+  try {
+    auto b = std::make_unique<TrajBatchHost>();
+    b->circuits.resize(num_circuits);
+    for (unsigned i = 0; i < num_circuits; ++i) {
       if (texts[i] == nullptr ||
           !BuildNoisy(texts[i], rows[i], cols[i], b->circuits[i])) {
         return nullptr;
       }
-    } catch (const std::exception&) {  // qsim's parser and allocations
-      return nullptr;
     }
+    return b.release();
+  } catch (const std::exception&) {  // qsim's parser and allocations
+    return nullptr;
   }
-  return b.release();
 }
 
 unsigned qsim_traj_num_channels(void* batch, unsigned circuit) {
@@ -655,9 +659,9 @@ unsigned qsim_traj_num_channels(void* batch, unsigned circuit) {
 
 namespace {
 
-int PlanBatch(TrajBatchHost* b, const unsigned* circuit_of,
-              const uint64_t* reps, unsigned num_traj,
-              const FusionSpec& spec) {
+int PlanBatchImpl(TrajBatchHost* b, const unsigned* circuit_of,
+                  const uint64_t* reps, unsigned num_traj,
+                  const FusionSpec& spec) {
   if (b == nullptr || circuit_of == nullptr || reps == nullptr ||
       num_traj == 0) {
     return QSIM_HIP_ERR_ARG;
@@ -780,6 +784,19 @@ int PlanBatch(TrajBatchHost* b, const unsigned* circuit_of,
                                pairs.data(), npairs, ndec, &b->handle);
 }
 
+// A failed host allocation while planning becomes an error return (an
+// exception may not cross the C ABI); the batch keeps no partial plan.
+int PlanBatch(TrajBatchHost* b, const unsigned* circuit_of,
+              const uint64_t* reps, unsigned num_traj,
+              const FusionSpec& spec) {
+  try {
+    return PlanBatchImpl(b, circuit_of, reps, num_traj, spec);
+  } catch (const std::bad_alloc&) {
+    if (b != nullptr) b->plans.clear();
+    return QSIM_HIP_ERR_RUNTIME;
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -817,8 +834,12 @@ int qsim_traj_run(void* batch, unsigned threads, int use_graph) {
 
 static int Fetch(TrajBatchHost* b) {
   if (b->have_results) return QSIM_HIP_OK;
-  b->zz.assign(b->total_pairs, 0.0);
-  b->dec.assign(b->total_decisions, -1);
+  try {
+    b->zz.assign(b->total_pairs, 0.0);
+    b->dec.assign(b->total_decisions, -1);
+  } catch (const std::bad_alloc&) {
+    return QSIM_HIP_ERR_RUNTIME;
+  }
   int rc = qsim_hip_traj_results(b->handle, b->zz.data(), b->total_pairs,
                                  b->dec.data(), b->total_decisions);
   if (rc == QSIM_HIP_OK) b->have_results = true;

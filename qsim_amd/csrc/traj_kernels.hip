@@ -30,6 +30,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <memory>
 #include <atomic>
 #include <cmath>
 #include <mutex>
@@ -837,11 +838,11 @@ using namespace qhip;
 
 extern "C" {
 
-int qsim_hip_traj_prepare(const qsim_hip_traj_desc* trajs, unsigned num_traj,
-                          const qsim_hip_traj_op* ops, uint64_t num_ops,
-                          const float* matrices, uint64_t num_floats,
-                          const uint32_t* pairs, uint32_t num_pairs,
-                          uint32_t num_decisions, void** handle) {
+static int TrajPrepare(const qsim_hip_traj_desc* trajs, unsigned num_traj,
+                       const qsim_hip_traj_op* ops, uint64_t num_ops,
+                       const float* matrices, uint64_t num_floats,
+                       const uint32_t* pairs, uint32_t num_pairs,
+                       uint32_t num_decisions, void** handle) {
   if (handle == nullptr || trajs == nullptr || num_traj == 0 ||
       (num_ops && ops == nullptr) || (num_floats && matrices == nullptr) ||
       (num_pairs && pairs == nullptr)) {
@@ -947,7 +948,8 @@ int qsim_hip_traj_prepare(const qsim_hip_traj_desc* trajs, unsigned num_traj,
     QHIP_TRY(BuildTilePasses(jobs, matrices, passes, recs));
     if (!passes.empty()) QHIP_TRY(TileKernelReady());
   }
-  auto* b = new TrajBatch;
+  // Released (device buffers included) on every early return or exception.
+  std::unique_ptr<TrajBatch, void (*)(TrajBatch*)> b(new TrajBatch, Release);
   b->num_traj = num_traj;
   for (const TrajTilePass& tp : passes) {
     b->max_tile_bits = std::max(b->max_tile_bits, tp.g.tb);
@@ -973,12 +975,27 @@ int qsim_hip_traj_prepare(const qsim_hip_traj_desc* trajs, unsigned num_traj,
       e = hipMalloc(&b->dec, std::max<size_t>(num_decisions, 1) * 4);
     if (e != hipSuccess) rc = HipFail(e, "trajectory batch buffers");
   }
-  if (rc != QSIM_HIP_OK) {
-    Release(b);
-    return rc;
-  }
-  *handle = b;
+  if (rc != QSIM_HIP_OK) return rc;
+  *handle = b.release();
   return QSIM_HIP_OK;
+}
+
+// A failed host allocation (the validated copies of the streams, the tile
+// passes, the batch record) becomes an error return: an exception may not
+// cross the C ABI.
+int qsim_hip_traj_prepare(const qsim_hip_traj_desc* trajs, unsigned num_traj,
+                          const qsim_hip_traj_op* ops, uint64_t num_ops,
+                          const float* matrices, uint64_t num_floats,
+                          const uint32_t* pairs, uint32_t num_pairs,
+                          uint32_t num_decisions, void** handle) {
+  try {
+    return TrajPrepare(trajs, num_traj, ops, num_ops, matrices, num_floats,
+                       pairs, num_pairs, num_decisions, handle);
+  } catch (const std::bad_alloc&) {
+    if (handle) *handle = nullptr;
+    SetError("traj_prepare: out of host memory");
+    return QSIM_HIP_ERR_RUNTIME;
+  }
 }
 
 int qsim_hip_traj_run(void* handle, unsigned threads, int use_graph) {
