@@ -15,9 +15,11 @@ apply loop over the gfx950 kernels.  The paths run on the same circuit:
                      for k>=5 passes
   blocked-13         cache-blocked passes over 13 tile qubits (5 lowest + 8
                      chosen), gates fused to <= 4 qubits in FP64 inside each
-                     pass, split-BF16 MFMA on the LDS-resident tile
-  blocked-14         the same over 14 tile qubits (5 lowest + 9 chosen; one
+                     pass, FP16 MFMA with a power-of-two scale per tile on
+                     the LDS-resident tile
+  blocked-14         the same over 14 tile qubits (4 lowest + 10 chosen; one
                      128 KiB tile per CU, fewer passes)
+  blocked-14-bf16    blocked-14 on the split-BF16 product instead of FP16
   learned            the partition the learned planner picks for this
                      circuit with the committed cost model
                      (qsim_amd/data/cost_model.json), among qsim's fuser at
@@ -84,12 +86,16 @@ SWEEP_PATHS = [f"fp32-k{k}" for k in range(2, 7)]
 # HBM bytes per amplitude per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE
 # runs, scaled by what the counters read for a copy of known size (reads x 2,
 # writes x 1); profiles/gate_passes_32q_blocked_vs_fused.md.
-MEASURED_HBM_BYTES_PER_AMP = {"GateBlockKernel": 16.04}  # 14-qubit tiles
+MEASURED_HBM_BYTES_PER_AMP = {"GateBlockF16Kernel": 16.04,  # 14-qubit tiles
+                              "GateBlockKernel": 16.04}
 
-# One 32-group batch of a 4-qubit op in a blocked pass: 8 MFMAs of
-# 32 x 32 x 16 (2 K blocks x 4 split products).
-BLOCK_FLOPS_PER_AMP_OP = 8 * 2 * 32 * 32 * 16 / 512.0
-MFMA_BF16_PEAK_TFLOPS = 2500.0
+# MFMA flops issued per amplitude per 4-qubit op of a blocked pass: a batch
+# of 32 groups (512 amplitudes) takes 2 K blocks x (3 FP16 or 4 split-BF16
+# products) MFMAs of 32 x 32 x 16.
+BLOCK_FLOPS_PER_AMP_OP = {"GateBlockF16Kernel": 6 * 2 * 32 * 32 * 16 / 512.0,
+                          "GateBlockKernel": 8 * 2 * 32 * 32 * 16 / 512.0}
+# Dense MFMA peak of MI355X, fp16 and bf16 alike.
+MFMA_F16_PEAK_TFLOPS = 2500.0
 
 
 def log(*a):
@@ -122,12 +128,14 @@ def available_paths():
             (PATHS[p][0] != "learned" or os.path.exists(costmodel.MODEL_PATH))]
 
 
-def kernel_family(circ, i, qubits, method, n):
+def kernel_family(circ, i, qubits, method, n, arith="f16"):
     """The gate kernel a pass runs (qsim_amd/csrc: gate_kernels.hip,
-    mfma_kernels.hip and block_kernels.hip dispatch)."""
+    mfma_kernels.hip, f16_kernels.hip and block_kernels.hip dispatch).
+    Blocked passes of unitary gates run GateBlockF16Kernel unless the
+    split-BF16 arithmetic is asked for."""
     from qsim_amd import driver
     if circ.pass_kind(i) == driver.PASS_BLOCKED:
-        return "GateBlockKernel"
+        return "GateBlockF16Kernel" if arith == "f16" else "GateBlockKernel"
     k = len(qubits)
     low = bool(qubits) and min(qubits) < 5
     if method != 1 and 4 <= k <= 6 and n >= k + 5:
@@ -143,7 +151,7 @@ def run_path(args, text, name):
     import qsim_amd
     from qsim_amd import driver
 
-    part, kmax, f64m, _, _ = PATHS[name]
+    part, kmax, f64m, _, arith = PATHS[name]
     circ = driver.Circuit(text)
     t0 = time.time()
     npass, method, pick = fuse(circ, name)
@@ -192,24 +200,25 @@ def run_path(args, text, name):
         f"(wall {wall:.3f}s), {len(pass_ms)} passes timed, norm {norm:.9f}")
 
     bytes_per_pass = 16.0 * (1 << n)   # every complex64 read and written once
-    fams = [kernel_family(circ, i, q, method, n)
+    fams = [kernel_family(circ, i, q, method, n, arith)
             for i, (q, _) in enumerate(passes)] * args.steps
     per_fam = {}
     flops = {}
     for i, (f, ms) in enumerate(zip(fams, pass_ms)):
         per_fam.setdefault(f, []).append(float(ms))
         flops.setdefault(f, []).append(
-            BLOCK_FLOPS_PER_AMP_OP * block_ops[i % npass] * (1 << n))
+            BLOCK_FLOPS_PER_AMP_OP.get(f, 0.0) * block_ops[i % npass]
+            * (1 << n))
     fam_stats = {}
     for f, v in per_fam.items():
         avg = float(np.mean(v))
         fam_stats[f] = {"passes": len(v), "avg_ms": avg,
                         "total_ms": float(np.sum(v)),
                         "achieved_gbps": bytes_per_pass / (avg * 1e-3) / 1e9}
-        if f == "GateBlockKernel":
+        if f in BLOCK_FLOPS_PER_AMP_OP:
             tf = float(np.sum(flops[f])) / (float(np.sum(v)) * 1e-3) / 1e12
             fam_stats[f]["mfma_tflops"] = tf
-            fam_stats[f]["mfma_frac"] = tf / MFMA_BF16_PEAK_TFLOPS
+            fam_stats[f]["mfma_frac"] = tf / MFMA_F16_PEAK_TFLOPS
             fam_stats[f]["avg_ops_per_pass"] = (
                 float(np.mean([o for o in block_ops if o])))
     dominant = max(fam_stats, key=lambda f: fam_stats[f]["total_ms"])

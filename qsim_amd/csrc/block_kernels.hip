@@ -189,7 +189,9 @@ GateBlockKernel(float2* __restrict__ s, BlockGeom g, BlockProgram p,
 // the wait for the next tile's loads (issued in full at the start of the
 // current tile's ops) never stands behind a record fetch issued after the
 // previous tile's stores.  The stores of a tile drain while the next
-// tile's ops run.  The tile is scaled by 2^e when staged and by 2^-e when
+// tile's ops run.  Measured on MI355X, an op costs about 2 ms per 32-qubit
+// pass whatever the HBM traffic: rewriting the tile in LDS (8-byte LDS
+// writes run at half the read rate) bounds it, not the MFMA or VALU work.  The tile is scaled by 2^e when staged and by 2^-e when
 // stored (TileScale, exact); one more barrier per tile for the maximum.
 template <int TB, bool FULL>
 __global__ void __launch_bounds__(BlockShape<TB>::NT)
@@ -230,25 +232,28 @@ GateBlockF16Kernel(float2* __restrict__ s, BlockGeom g, BlockProgram p,
     const uint64_t base = InsertZeros(tile << g.Lc, g.outer, g.nouter);
     // The next tile of this workgroup, or this one again past the end (an
     // in-bounds read that is never used), so the prefetch is unconditional.
+    // Its loads go out in two halves, the second after the second op (or
+    // after a single op): a whole tile's loads at once exceed what a CU keeps
+    // in flight, and the waves would stall issuing them instead of running
+    // the ops.
     const uint64_t next = tile + gridDim.x < g.ntiles ? tile + gridDim.x : tile;
-    io.template Fetch<FULL, 15u>(
-        s, InsertZeros(next << g.Lc, g.outer, g.nouter), g, buf);
+    const uint64_t nb = InsertZeros(next << g.Lc, g.outer, g.nouter);
+    io.template Fetch<FULL, 3u>(s, nb, g, buf);
     for (unsigned op = 0; op < p.nops; ++op) {
       const uint32_t* r = rec + op * unsigned(kF16RecWords);
       v8h f[4];
       LoadFragF16(r, l, f);
       const uint32_t* tab = r + kF16TabOff + 16 * (l >> 5);
       const uint32_t sj = r[kF16TabOff + kTableWords + (l & 31)];
-      uint32_t rd[8], wr[8];
+      uint32_t mo[8];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        rd[q] = tab[q] ^ sj;
-        wr[q] = tab[8 + q] ^ sj;
-      }
+      for (int q = 0; q < 8; ++q) mo[q] = tab[q] ^ sj;
       ApplyTileOpF16(reinterpret_cast<const uint8_t*>(r + kF16BposOff), f,
-                         rd, wr, nbatch, wave, NW);
+                     mo, nbatch, wave, NW);
       __syncthreads();
+      if (op == 1) io.template Fetch<FULL, 12u>(s, nb, g, buf);
     }
+    if (p.nops == 1) io.template Fetch<FULL, 12u>(s, nb, g, buf);
     io.template StoreScaled<16, FULL>(s, base, g, lds, e);
     e = TileScale<NW>(LocalMax<PER, FULL>(buf, nper), slots, p.headroom);
     io.template PutScaled<FULL>(g, buf, lds, e);
@@ -426,12 +431,25 @@ double NormBound16(const double* u) {
   return std::sqrt(best);
 }
 
-// FP16 op record of the padded matrix u (FP64) and the tables.
+// The member a lane half h reads as the q-th B value (record column order)
+// is the one its q-th C pair holds.
+constexpr bool F16ReadsAreWrites() {
+  for (unsigned h = 0; h < 2; ++h)
+    for (unsigned q = 0; q < 8; ++q)
+      if (F16Col(4 * h + 8 * (q >> 2) + (q & 3)) != 2 * h + 4 * (q >> 1) + (q & 1))
+        return false;
+  return true;
+}
+static_assert(F16ReadsAreWrites(), "FP16 member order");
+
+// FP16 op record of the padded matrix u (FP64) and the tables: record
+// column p of row r holds u[r][F16Col(p)].
 void BuildRecF16(const double* u, const uint32_t* tab, uint32_t* rec) {
   for (unsigned e = 0; e < 256; ++e) {
-    const uint16_t hr = HalfBits(u[2 * e]), hi = HalfBits(u[2 * e + 1]);
-    const uint16_t lr = HalfBits(u[2 * e] - HalfValue(hr));
-    const uint16_t li = HalfBits(u[2 * e + 1] - HalfValue(hi));
+    const unsigned src = (e & ~15u) | F16Col(e & 15u);
+    const uint16_t hr = HalfBits(u[2 * src]), hi = HalfBits(u[2 * src + 1]);
+    const uint16_t lr = HalfBits(u[2 * src] - HalfValue(hr));
+    const uint16_t li = HalfBits(u[2 * src + 1] - HalfValue(hi));
     rec[e] = uint32_t(hr) | (uint32_t(hi) << 16);
     rec[kF16MatWords + e] = uint32_t(lr) | (uint32_t(li) << 16);
   }
@@ -544,7 +562,9 @@ int PrepareGateBlock(unsigned n, const unsigned* tile, unsigned nt,
     }
     // Lane tables: member m of a group sits at tile offset toff(m) (bit b of
     // m at position pad[b]); lane half h reads members 4h + 8(q/4) + q%4 and
-    // writes members 2h + 4q + u (the operand maps of the MFMA product).
+    // writes members 2h + 4(q/2) + q%2 (the operand maps of the MFMA
+    // product).  FP16 records store the matrix columns permuted (F16Col),
+    // which makes the members read those written.
     auto toff = [&](unsigned m) {
       unsigned o = 0;
       for (int b = 0; b < 4; ++b) o |= ((m >> b) & 1u) << pad[b];
@@ -553,8 +573,10 @@ int PrepareGateBlock(unsigned n, const unsigned* tile, unsigned nt,
     uint32_t tab[kTableWords + kLaneWords];
     for (unsigned h = 0; h < 2; ++h) {
       for (unsigned q = 0; q < 8; ++q) {
-        tab[16 * h + q] = uint32_t(sizeof(float2)) *
-                          HostTileSwz(toff(4 * h + 8 * (q >> 2) + (q & 3)));
+        const unsigned rm = arith == kBlockArithF16
+                                ? F16Col(4 * h + 8 * (q >> 2) + (q & 3))
+                                : 4 * h + 8 * (q >> 2) + (q & 3);
+        tab[16 * h + q] = uint32_t(sizeof(float2)) * HostTileSwz(toff(rm));
         tab[16 * h + 8 + q] = uint32_t(sizeof(float2)) *
                               HostTileSwz(toff(2 * h + 4 * (q >> 1) + (q & 1)));
       }

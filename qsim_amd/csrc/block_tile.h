@@ -247,14 +247,25 @@ typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 typedef _Float16 v2h __attribute__((ext_vector_type(2)));
 
 // FP16 op record, 32-bit words: the 16 x 16 complex matrix as packed
-// (re, im) fp16 pairs at word 16 r + c, hi part then lo part; the member
+// (re, im) fp16 pairs, row r at words 16 r .. 16 r + 15 with record column
+// p holding matrix column F16Col(p), hi part then lo part; the member
 // tables, the group table and the batch-bit positions as in the bf16
-// record; padded to a multiple of 4 words.
+// record; padded to a multiple of 4 words.  With that column order the 8
+// members a lane reads for the B operand are the 8 members its C registers
+// hold, in the same order (both: 8 kb + 4 (c / 2) + 2 h + c % 2 for
+// q = 4 kb + c), so one LDS address per member serves the read and the
+// write of a batch (the read and write halves of the member table are the
+// same).
 constexpr int kF16MatWords = 256;
 constexpr int kF16TabOff = 2 * kF16MatWords;
 constexpr int kF16BposOff = kF16TabOff + kTableWords + kLaneWords;
 constexpr int kF16RecWords = kF16BposOff + 4;  // 580
 static_assert(kF16RecWords % 4 == 0, "records are copied as uint4");
+
+// Matrix column stored at record column p: p with bits 1 and 2 swapped.
+__host__ __device__ constexpr unsigned F16Col(unsigned p) {
+  return (p & 9u) | ((p & 2u) << 1) | ((p & 4u) >> 1);
+}
 
 // LDS of an FP16 blocked-pass workgroup: the tile, 16 floats for the
 // per-wave maxima, then the pass's op records (all of them, loaded once per
@@ -273,8 +284,9 @@ struct F16Lds {
 
 // Lane l's A fragments of the op whose record is at r (LDS): for K block
 // kb, f[2 kb] = hi and f[2 kb + 1] = lo of A[l%32][16 kb + 8 (l/32) + e].
-// Row l%32 = 2 i + p reads U[i][8 kb + 4 (l/32) + c], c < 4, and forms
-// (Re, -Im) for p = 0 or (Im, Re) for p = 1 from the stored (Re, Im) pair.
+// Row l%32 = 2 i + p reads record columns 8 kb + 4 (l/32) + c, c < 4, of
+// row i, and forms (Re, -Im) for p = 0 or (Im, Re) for p = 1 from the
+// stored (Re, Im) pair.
 __device__ __forceinline__ void LoadFragF16(const uint32_t* r, int l,
                                             v8h (&f)[4]) {
   const unsigned i = unsigned(l & 31) >> 1, hh = unsigned(l) >> 5;
@@ -295,22 +307,22 @@ __device__ __forceinline__ void LoadFragF16(const uint32_t* r, int l,
     }
 }
 
-// Lane half h's 8 members as fp16 (hi, lo) pairs: hi rounds to nearest, the
-// residual v - hi is exact in FP32 (one v_fma_mix_f32 from the fp16 half)
-// and rounds to lo.
+// Lane half h's 8 members as fp16 (hi, lo) pairs: hi rounds to nearest;
+// the residual v - hi is exact in FP32 and v_fma_mix{lo,hi}_f16 round it
+// to fp16 straight from the fp16 half, one instruction per value.
 __device__ __forceinline__ void SplitF16(const v2f (&v)[8], v8h (&bh)[2],
                                          v8h (&bl)[2]) {
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
     const v2h hi = __builtin_convertvector(v[q], v2h);
-    float rx, ry;
-    asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]"
-        : "=v"(rx)
+    uint32_t lw;
+    asm("v_fma_mixlo_f16 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]"
+        : "=v"(lw)
         : "v"(hi), "v"(v[q][0]));
-    asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]"
-        : "=v"(ry)
+    asm("v_fma_mixhi_f16 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]"
+        : "+v"(lw)
         : "v"(hi), "v"(v[q][1]));
-    const v2h lo = __builtin_convertvector((v2f{rx, ry}), v2h);
+    const v2h lo = __builtin_bit_cast(v2h, lw);
     bh[q >> 2][2 * (q & 3)] = hi[0];
     bh[q >> 2][2 * (q & 3) + 1] = hi[1];
     bl[q >> 2][2 * (q & 3)] = lo[0];
@@ -332,12 +344,24 @@ __device__ __forceinline__ v16f MfmaChainF16(const v8h (&f)[4],
   return c;
 }
 
+// Results of a batch back to the slots its inputs came from: C registers
+// (4 q + 2 u, 4 q + 2 u + 1) are member index 2 q + u of the lane's list.
+__device__ __forceinline__ void WriteBack(const uint32_t (&a)[8],
+                                          const v16f& c) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      *LdsAt(a[2 * q + u]) = v2f{c[4 * q + 2 * u], c[4 * q + 2 * u + 1]};
+    }
+}
+
 // One FP16 op over the (scaled) LDS tile; batches as in ApplyTileOp, two
-// at a time.
+// at a time.  mo: the lane's member offsets (record column order, so that
+// reads and writes share them) XOR its group's slot.
 __device__ __forceinline__ void ApplyTileOpF16(const uint8_t* bp,
                                                const v8h (&f)[4],
-                                               const uint32_t (&rd)[8],
-                                               const uint32_t (&wr)[8],
+                                               const uint32_t (&mo)[8],
                                                unsigned nbatch, unsigned wave,
                                                unsigned nwaves) {
   unsigned b[5];
@@ -348,27 +372,38 @@ __device__ __forceinline__ void ApplyTileOpF16(const uint8_t* bp,
   for (; bt + nwaves < nbatch; bt += 2 * nwaves) {
     const unsigned sp0 = BatchSlot(bt, b);
     const unsigned sp1 = BatchSlot(bt + nwaves, b);
+    uint32_t a0[8], a1[8];
     v2f v0[8], v1[8];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) v0[q] = *LdsAt(sp0 ^ rd[q]);
+    for (int q = 0; q < 8; ++q) {
+      a0[q] = sp0 ^ mo[q];
+      v0[q] = *LdsAt(a0[q]);
+    }
     v8h bh0[2], bl0[2], bh1[2], bl1[2];
     SplitF16(v0, bh0, bl0);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) v1[q] = *LdsAt(sp1 ^ rd[q]);
+    for (int q = 0; q < 8; ++q) {
+      a1[q] = sp1 ^ mo[q];
+      v1[q] = *LdsAt(a1[q]);
+    }
     const v16f c0 = MfmaChainF16(f, bh0, bl0);
     SplitF16(v1, bh1, bl1);
     const v16f c1 = MfmaChainF16(f, bh1, bl1);
-    WriteBatch(sp0, wr, c0);
-    WriteBatch(sp1, wr, c1);
+    WriteBack(a0, c0);
+    WriteBack(a1, c1);
   }
   if (bt < nbatch) {
     const unsigned sp0 = BatchSlot(bt, b);
+    uint32_t a0[8];
     v2f v0[8];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) v0[q] = *LdsAt(sp0 ^ rd[q]);
+    for (int q = 0; q < 8; ++q) {
+      a0[q] = sp0 ^ mo[q];
+      v0[q] = *LdsAt(a0[q]);
+    }
     v8h bh0[2], bl0[2];
     SplitF16(v0, bh0, bl0);
-    WriteBatch(sp0, wr, MfmaChainF16(f, bh0, bl0));
+    WriteBack(a0, MfmaChainF16(f, bh0, bl0));
   }
 }
 
