@@ -86,7 +86,7 @@ SWEEP_PATHS = [f"fp32-k{k}" for k in range(2, 7)]
 # HBM bytes per amplitude per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE
 # runs, scaled by what the counters read for a copy of known size (reads x 2,
 # writes x 1); profiles/gate_passes_32q_blocked_vs_fused.md.
-MEASURED_HBM_BYTES_PER_AMP = {"GateBlockF16Kernel": 16.04,  # 14-qubit tiles
+MEASURED_HBM_BYTES_PER_AMP = {"GateBlockF16Kernel": 16.03,  # 14-qubit tiles
                               "GateBlockKernel": 16.04}
 
 # MFMA flops issued per amplitude per 4-qubit op of a blocked pass: a batch
