@@ -22,12 +22,12 @@
 namespace qsim {
 namespace hip {
 
-constexpr unsigned kCostFeatures = 18;
+constexpr unsigned kCostFeatures = 19;
 constexpr unsigned kCostHidden = 64;
 
 // Kernel family a pass runs on (gate_kernels.hip / mfma_kernels.hip /
-// block_kernels.hip dispatch).
-enum KernelFamily { kVector = 0, kMfmaF32 = 1, kMfmaBf16 = 2, kBlocked = 3 };
+// f16_kernels.hip dispatch).
+enum KernelFamily { kVector = 0, kMfmaF32 = 1, kMfmaBf16 = 2, kMfmaF16 = 3 };
 
 inline int GateFamily(unsigned k, unsigned num_controls, int method,
                       unsigned n) {
@@ -36,6 +36,11 @@ inline int GateFamily(unsigned k, unsigned num_controls, int method,
     return kVector;
   }
   if (method == QSIM_HIP_METHOD_MFMA_BF16X3 && k >= 5) return kMfmaBf16;
+  // The FP16 kernels take 5- and 6-qubit gates on 12 qubits or more (the
+  // FP32 MFMA kernels run the rest).
+  if (method == QSIM_HIP_METHOD_MFMA_FP16X2 && k >= 5 && n >= 12) {
+    return kMfmaF16;
+  }
   return kMfmaF32;
 }
 
@@ -48,8 +53,8 @@ inline float CostTargetOffset(unsigned n) {
 //   0 n, 1 blocked flag, 2 k, 3..8 sorted targets / n (0 past k),
 //   9 targets below qubit 5 (the LDS-tile boundary of the gate kernels),
 //   10 targets below qubit 13, 11 controlled flag, 12 controls,
-//   13 diagonal flag, 14..16 family one-hot (vector, FP32 MFMA, BF16 MFMA),
-//   17 lowest target / n.
+//   13 diagonal flag, 14..17 family one-hot (vector, FP32 MFMA, BF16 MFMA,
+//   FP16 MFMA), 18 lowest target / n.
 inline void GateFeatures(unsigned n, const std::vector<unsigned>& qs,
                          unsigned num_controls, bool diagonal, int family,
                          float* f) {
@@ -68,13 +73,13 @@ inline void GateFeatures(unsigned n, const std::vector<unsigned>& qs,
   f[11] = num_controls > 0 ? 1.0f : 0.0f;
   f[12] = float(num_controls);
   f[13] = diagonal ? 1.0f : 0.0f;
-  f[14 + std::min(family, 2)] = 1.0f;
-  f[17] = qs.empty() ? 0.0f : float(qs[0]) / fn;
+  f[14 + std::min(family, 3)] = 1.0f;
+  f[18] = qs.empty() ? 0.0f : float(qs[0]) / fn;
 }
 
 // Feature vector of a cache-blocked pass: 0 n, 1 blocked flag, 2 tile
 // qubits, 3..8 the lowest six tile qubits above the low run / n,
-// 9 low-run length, 10 fused gates in the pass, 17 lowest high qubit / n.
+// 9 low-run length, 10 fused gates in the pass, 18 lowest high qubit / n.
 inline void BlockFeatures(unsigned n, const std::vector<unsigned>& tile,
                           unsigned num_ops, float* f) {
   std::fill(f, f + kCostFeatures, 0.0f);
@@ -89,7 +94,7 @@ inline void BlockFeatures(unsigned n, const std::vector<unsigned>& tile,
   }
   f[9] = float(lc);
   f[10] = float(num_ops);
-  f[17] = lc < tile.size() ? float(tile[lc]) / fn : 1.0f;
+  f[18] = lc < tile.size() ? float(tile[lc]) / fn : 1.0f;
 }
 
 // A candidate partition: qsim's fuser at max_fused_size `a` with gate method
@@ -106,6 +111,7 @@ inline std::vector<PlanCandidate> DefaultCandidates() {
     c.push_back({0, k, 0, QSIM_HIP_METHOD_AUTO});
     if (k >= 4) c.push_back({0, k, 0, QSIM_HIP_METHOD_VECTOR});
     if (k >= 5) c.push_back({0, k, 0, QSIM_HIP_METHOD_MFMA_BF16X3});
+    if (k >= 5) c.push_back({0, k, 0, QSIM_HIP_METHOD_MFMA_FP16X2});
   }
   const unsigned shapes[][2] = {{14, 4}, {14, 5}, {14, 6}, {13, 4},
                                 {13, 5}, {13, 6}, {12, 5}, {11, 4}};

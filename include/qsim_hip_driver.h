@@ -83,7 +83,7 @@ int qsim_driver_set_method(void* circuit, int method);
 enum { QSIM_DRIVER_BLOCK_F16 = 0, QSIM_DRIVER_BLOCK_BF16 = 1 };
 int qsim_driver_set_block_arith(void* circuit, int arith);
 
-/* Cost-model features (include/qsim_amd/learned_planner.h, 18 floats) of
+/* Cost-model features (include/qsim_amd/learned_planner.h, 19 floats) of
  * pass i of the current partition when run with gate method `method`. */
 int qsim_driver_pass_features(void* circuit, unsigned i, int method,
                               float* out);

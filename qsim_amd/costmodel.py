@@ -31,7 +31,7 @@ from . import F32, PassTimer, StateSpace, _check, lib, synchronize
 HERE = os.path.dirname(os.path.abspath(__file__))
 MODEL_PATH = os.path.join(HERE, "data", "cost_model.json")
 
-NUM_FEATURES = 18
+NUM_FEATURES = 19
 HIDDEN = 64
 
 
@@ -172,7 +172,7 @@ def candidates():
 def collect(sizes, seeds, cycles=(20,), cands=None, log=None):
     """Per-pass (features, target) pairs: every candidate partition of seeded
     grid circuits, timed pass by pass with HIP events (one warm-up run, one
-    timed run).  Returns X (N, 18), y (N,), and a record per run."""
+    timed run).  Returns X (N, 19), y (N,), and a record per run."""
     from . import circuits, driver
     cands = cands or candidates()
     xs, ys, runs = [], [], []

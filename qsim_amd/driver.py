@@ -21,7 +21,7 @@ FUSER_MQUBIT = 0
 FUSER_LEARNED = 1
 FUSER_NONE = 2
 
-NUM_FEATURES = 18  # include/qsim_amd/learned_planner.h
+NUM_FEATURES = 19  # include/qsim_amd/learned_planner.h
 
 PASS_GATE = 0
 PASS_MEASUREMENT = 1
@@ -118,7 +118,7 @@ class Circuit:
         _check(driver_lib().qsim_driver_set_method(self.h, method))
 
     def pass_features(self, method: int) -> np.ndarray:
-        """Cost-model features of every pass, (num_passes, 18) float32."""
+        """Cost-model features of every pass, (num_passes, 19) float32."""
         out = np.zeros((self.num_passes, NUM_FEATURES), np.float32)
         for i in range(self.num_passes):
             _check(driver_lib().qsim_driver_pass_features(
