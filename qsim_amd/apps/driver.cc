@@ -10,6 +10,7 @@
 #include <cstdint>
 #include <cstring>
 #include <memory>
+#include <new>
 #include <random>
 #include <sstream>
 #include <string>
@@ -339,16 +340,31 @@ int FuseBlocked(DriverCircuit& d, unsigned tile_bits, unsigned low_bits,
   return rc;
 }
 
+// Runs an entry point's body: a failed host allocation becomes an error
+// return instead of an exception crossing the C boundary.
+template <typename F>
+int GuardHost(F&& f) {
+  try {
+    return f();
+  } catch (const std::bad_alloc&) {
+    return QSIM_HIP_ERR_RUNTIME;
+  }
+}
+
 }  // namespace
 
 extern "C" {
 
 void* qsim_driver_load(const char* text, unsigned maxtime) {
-  auto d = std::make_unique<DriverCircuit>();
-  std::string t(text);
-  if (!Parse<float>(t, maxtime, d->c32)) return nullptr;
-  if (!Parse<double>(t, maxtime, d->c64)) return nullptr;
-  return d.release();
+  try {
+    auto d = std::make_unique<DriverCircuit>();
+    std::string t(text);
+    if (!Parse<float>(t, maxtime, d->c32)) return nullptr;
+    if (!Parse<double>(t, maxtime, d->c64)) return nullptr;
+    return d.release();
+  } catch (const std::bad_alloc&) {
+    return nullptr;
+  }
 }
 
 void qsim_driver_free(void* c) { delete static_cast<DriverCircuit*>(c); }
@@ -365,26 +381,30 @@ void qsim_driver_set_cost_model(void* model) { g_cost_model = model; }
 
 int qsim_driver_fuse(void* c, unsigned max_fused_size, int fuser,
                      int matrix_prec) {
-  auto* d = static_cast<DriverCircuit*>(c);
-  d->method = -1;
-  if (fuser == QSIM_DRIVER_FUSER_LEARNED) {
-    return qsim_driver_plan_learned(c, g_cost_model, 0, nullptr, nullptr,
-                                    nullptr, nullptr, nullptr, nullptr,
-                                    nullptr, nullptr);
-  }
-  d->matrix_prec = matrix_prec;
-  int rc = Fuse(d->c32, max_fused_size, fuser, d->f32);
-  if (rc == QSIM_HIP_OK) rc = Fuse(d->c64, max_fused_size, fuser, d->f64);
-  if (rc != QSIM_HIP_OK) {
-    d->f32.clear();
-    d->f64.clear();
-  }
-  PlainSteps(*d);
-  return rc;
+  return GuardHost([&]() -> int {
+    auto* d = static_cast<DriverCircuit*>(c);
+    d->method = -1;
+    if (fuser == QSIM_DRIVER_FUSER_LEARNED) {
+      return qsim_driver_plan_learned(c, g_cost_model, 0, nullptr, nullptr,
+                                      nullptr, nullptr, nullptr, nullptr,
+                                      nullptr, nullptr);
+    }
+    d->matrix_prec = matrix_prec;
+    int rc = Fuse(d->c32, max_fused_size, fuser, d->f32);
+    if (rc == QSIM_HIP_OK) rc = Fuse(d->c64, max_fused_size, fuser, d->f64);
+    if (rc != QSIM_HIP_OK) {
+      d->f32.clear();
+      d->f64.clear();
+    }
+    PlainSteps(*d);
+    return rc;
+  });
 }
 
 int qsim_driver_fuse_blocked(void* c, unsigned tile_bits, unsigned low_bits) {
-  return FuseBlocked(*static_cast<DriverCircuit*>(c), tile_bits, low_bits, -1);
+  return GuardHost([&]() -> int {
+    return FuseBlocked(*static_cast<DriverCircuit*>(c), tile_bits, low_bits, -1);
+  });
 }
 
 unsigned qsim_driver_num_passes(void* c) {
@@ -443,26 +463,28 @@ int qsim_driver_block_info(void* c, unsigned i, unsigned* num_ops,
 
 int qsim_driver_run_passes(void* c, void* state, int prec, unsigned first,
                            unsigned count, unsigned seed) {
-  auto* d = static_cast<DriverCircuit*>(c);
-  if (state == nullptr || first > d->steps.size() ||
-      count > d->steps.size() - first || d->f32.size() != d->f64.size()) {
-    return QSIM_HIP_ERR_ARG;
-  }
-  if (prec == QSIM_HIP_F64) {
-    return RunF64(*d, static_cast<double*>(state), first, count, seed);
-  }
-  const int old_method = qsim_hip_get_gate_method();
-  if (d->method >= 0 && d->method != old_method) {
-    int rc = qsim_hip_set_gate_method(d->method);
-    if (rc != QSIM_HIP_OK) return rc;
-  }
-  int rc = d->matrix_prec == QSIM_HIP_F64
-               ? RunF32M64(*d, static_cast<float*>(state), first, count, seed)
-               : RunF32(*d, static_cast<float*>(state), first, count, seed);
-  if (d->method >= 0 && d->method != old_method) {
-    qsim_hip_set_gate_method(old_method);
-  }
-  return rc;
+  return GuardHost([&]() -> int {
+    auto* d = static_cast<DriverCircuit*>(c);
+    if (state == nullptr || first > d->steps.size() ||
+        count > d->steps.size() - first || d->f32.size() != d->f64.size()) {
+      return QSIM_HIP_ERR_ARG;
+    }
+    if (prec == QSIM_HIP_F64) {
+      return RunF64(*d, static_cast<double*>(state), first, count, seed);
+    }
+    const int old_method = qsim_hip_get_gate_method();
+    if (d->method >= 0 && d->method != old_method) {
+      int rc = qsim_hip_set_gate_method(d->method);
+      if (rc != QSIM_HIP_OK) return rc;
+    }
+    int rc = d->matrix_prec == QSIM_HIP_F64
+                 ? RunF32M64(*d, static_cast<float*>(state), first, count, seed)
+                 : RunF32(*d, static_cast<float*>(state), first, count, seed);
+    if (d->method >= 0 && d->method != old_method) {
+      qsim_hip_set_gate_method(old_method);
+    }
+    return rc;
+  });
 }
 
 int qsim_driver_set_method(void* c, int method) {
@@ -485,10 +507,12 @@ int qsim_driver_set_block_arith(void* c, int arith) {
 }
 
 int qsim_driver_pass_features(void* c, unsigned i, int method, float* out) {
-  auto* d = static_cast<DriverCircuit*>(c);
-  if (i >= d->steps.size() || out == nullptr) return QSIM_HIP_ERR_ARG;
-  PassFeatures(*d, i, method, out);
-  return QSIM_HIP_OK;
+  return GuardHost([&]() -> int {
+    auto* d = static_cast<DriverCircuit*>(c);
+    if (i >= d->steps.size() || out == nullptr) return QSIM_HIP_ERR_ARG;
+    PassFeatures(*d, i, method, out);
+    return QSIM_HIP_OK;
+  });
 }
 
 int qsim_driver_plan_learned(void* c, void* model, unsigned max_candidates,
@@ -496,43 +520,45 @@ int qsim_driver_plan_learned(void* c, void* model, unsigned max_candidates,
                              int* methods, double* predicted_ms,
                              unsigned* num_passes, unsigned* num_candidates,
                              int* chosen) {
-  auto* d = static_cast<DriverCircuit*>(c);
-  const unsigned n = d->c64.num_qubits;
-  const std::vector<hip::PlanCandidate> cands = hip::DefaultCandidates();
-  std::vector<hip::CandidateResult> res(cands.size());
-  int best = -1;
-  if (model != nullptr) {
-    int rc = QSIM_HIP_OK;
-    best = hip::ScoreCandidates<IO>(model, n, d->c64.ops, cands, res, &rc);
+  return GuardHost([&]() -> int {
+    auto* d = static_cast<DriverCircuit*>(c);
+    const unsigned n = d->c64.num_qubits;
+    const std::vector<hip::PlanCandidate> cands = hip::DefaultCandidates();
+    std::vector<hip::CandidateResult> res(cands.size());
+    int best = -1;
+    if (model != nullptr) {
+      int rc = QSIM_HIP_OK;
+      best = hip::ScoreCandidates<IO>(model, n, d->c64.ops, cands, res, &rc);
+      if (rc != QSIM_HIP_OK) return rc;
+    }
+    const hip::PlanCandidate pick =
+        best >= 0 ? cands[best] : hip::FallbackCandidate();
+    const int trial =
+        best >= 0 && pick.blocked ? int(res[best].plan.trial) : -1;
+    int rc = ApplyCandidate(*d, pick, trial);
     if (rc != QSIM_HIP_OK) return rc;
-  }
-  const hip::PlanCandidate pick =
-      best >= 0 ? cands[best] : hip::FallbackCandidate();
-  const int trial =
-      best >= 0 && pick.blocked ? int(res[best].plan.trial) : -1;
-  int rc = ApplyCandidate(*d, pick, trial);
-  if (rc != QSIM_HIP_OK) return rc;
-  if (best < 0) {
-    // No model: the fallback, reported by its index in the list.
-    for (size_t j = 0; j < cands.size(); ++j) {
-      if (!cands[j].blocked && cands[j].a == pick.a &&
-          cands[j].method == pick.method) {
-        best = int(j);
+    if (best < 0) {
+      // No model: the fallback, reported by its index in the list.
+      for (size_t j = 0; j < cands.size(); ++j) {
+        if (!cands[j].blocked && cands[j].a == pick.a &&
+            cands[j].method == pick.method) {
+          best = int(j);
+        }
       }
     }
-  }
-  const unsigned m = std::min<unsigned>(max_candidates, cands.size());
-  for (unsigned j = 0; j < m; ++j) {
-    if (blocked) blocked[j] = cands[j].blocked;
-    if (a) a[j] = cands[j].a;
-    if (b) b[j] = cands[j].b;
-    if (methods) methods[j] = cands[j].method;
-    if (predicted_ms) predicted_ms[j] = res[j].predicted_ms;
-    if (num_passes) num_passes[j] = res[j].passes;
-  }
-  if (num_candidates) *num_candidates = cands.size();
-  if (chosen) *chosen = best;
-  return QSIM_HIP_OK;
+    const unsigned m = std::min<unsigned>(max_candidates, cands.size());
+    for (unsigned j = 0; j < m; ++j) {
+      if (blocked) blocked[j] = cands[j].blocked;
+      if (a) a[j] = cands[j].a;
+      if (b) b[j] = cands[j].b;
+      if (methods) methods[j] = cands[j].method;
+      if (predicted_ms) predicted_ms[j] = res[j].predicted_ms;
+      if (num_passes) num_passes[j] = res[j].passes;
+    }
+    if (num_candidates) *num_candidates = cands.size();
+    if (chosen) *chosen = best;
+    return QSIM_HIP_OK;
+  });
 }
 
 int qsim_driver_run(void* c, void* state, int prec, unsigned seed) {
@@ -541,37 +567,39 @@ int qsim_driver_run(void* c, void* state, int prec, unsigned seed) {
 }
 
 int qsim_driver_capture(void* c, void* state, int prec, void** plan) {
-  if (c == nullptr || state == nullptr || plan == nullptr) {
-    return QSIM_HIP_ERR_ARG;
-  }
-  *plan = nullptr;
-  const unsigned np = qsim_driver_num_passes(c);
-  for (unsigned i = 0; i < np; ++i) {
-    if (qsim_driver_pass_kind(c, i) == QSIM_DRIVER_PASS_MEASUREMENT) {
-      return QSIM_HIP_ERR_UNSUPPORTED;
+  return GuardHost([&]() -> int {
+    if (c == nullptr || state == nullptr || plan == nullptr) {
+      return QSIM_HIP_ERR_ARG;
     }
-  }
-  auto* d = static_cast<DriverCircuit*>(c);
-  void* p = nullptr;
-  int rc = qsim_hip_graph_record_begin(&p);
-  if (rc != QSIM_HIP_OK) return rc;
-  d->capturing = true;
-  rc = qsim_driver_run(c, state, prec, 0);
-  int rc2 = qsim_hip_graph_record_end(p);
-  if (rc == QSIM_HIP_OK) rc = rc2;
-  if (rc == QSIM_HIP_OK) rc = qsim_hip_graph_capture_begin(p);
-  if (rc == QSIM_HIP_OK) {
+    *plan = nullptr;
+    const unsigned np = qsim_driver_num_passes(c);
+    for (unsigned i = 0; i < np; ++i) {
+      if (qsim_driver_pass_kind(c, i) == QSIM_DRIVER_PASS_MEASUREMENT) {
+        return QSIM_HIP_ERR_UNSUPPORTED;
+      }
+    }
+    auto* d = static_cast<DriverCircuit*>(c);
+    void* p = nullptr;
+    int rc = qsim_hip_graph_record_begin(&p);
+    if (rc != QSIM_HIP_OK) return rc;
+    d->capturing = true;
     rc = qsim_driver_run(c, state, prec, 0);
-    rc2 = qsim_hip_graph_capture_end(p);
+    int rc2 = qsim_hip_graph_record_end(p);
     if (rc == QSIM_HIP_OK) rc = rc2;
-  }
-  d->capturing = false;
-  if (rc != QSIM_HIP_OK) {
-    qsim_hip_graph_free(p);
-    return rc;
-  }
-  *plan = p;
-  return QSIM_HIP_OK;
+    if (rc == QSIM_HIP_OK) rc = qsim_hip_graph_capture_begin(p);
+    if (rc == QSIM_HIP_OK) {
+      rc = qsim_driver_run(c, state, prec, 0);
+      rc2 = qsim_hip_graph_capture_end(p);
+      if (rc == QSIM_HIP_OK) rc = rc2;
+    }
+    d->capturing = false;
+    if (rc != QSIM_HIP_OK) {
+      qsim_hip_graph_free(p);
+      return rc;
+    }
+    *plan = p;
+    return QSIM_HIP_OK;
+  });
 }
 
 }  // extern "C"

@@ -564,18 +564,20 @@ int qsim_hip_sample(int prec, const void* s, unsigned n, const double* draws,
 
 int qsim_hip_sample_seeded(int prec, const void* s, unsigned n, uint64_t num,
                            unsigned seed, uint64_t* out) {
-  if (num == 0) return CheckState(prec, s, n);
-  double norm = 0;
-  QHIP_TRY(qsim_hip_norm(prec, s, n, &norm));
-  // The draws of lib/util.h:67-85 (mt19937, uniform_real_distribution, sort).
-  std::vector<double> rs;
-  rs.reserve(num + 1);
-  std::mt19937 rgen(seed);
-  std::uniform_real_distribution<double> distr(0.0, norm);
-  for (uint64_t i = 0; i < num; ++i) rs.push_back(distr(rgen));
-  std::sort(rs.begin(), rs.end());
-  rs.push_back(norm);
-  return qsim_hip_sample(prec, s, n, rs.data(), num, out);
+  return GuardHost([&]() -> int {
+    if (num == 0) return CheckState(prec, s, n);
+    double norm = 0;
+    QHIP_TRY(qsim_hip_norm(prec, s, n, &norm));
+    // The draws of lib/util.h:67-85 (mt19937, uniform_real_distribution, sort).
+    std::vector<double> rs;
+    rs.reserve(num + 1);
+    std::mt19937 rgen(seed);
+    std::uniform_real_distribution<double> distr(0.0, norm);
+    for (uint64_t i = 0; i < num; ++i) rs.push_back(distr(rgen));
+    std::sort(rs.begin(), rs.end());
+    rs.push_back(norm);
+    return qsim_hip_sample(prec, s, n, rs.data(), num, out);
+  });
 }
 
 unsigned qsim_hip_num_partial_norms(unsigned n) { return PartialNormCount(n); }
@@ -597,9 +599,11 @@ int qsim_hip_find_measured_bits(int prec, const void* s, unsigned n,
 
 int qsim_hip_collapse(int prec, void* s, unsigned n, uint64_t mask,
                       uint64_t bits) {
-  QHIP_TRY(CheckState(prec, s, n));
-  return QHIP_DISPATCH(prec, Collapse<float>((float2*)s, n, mask, bits),
-                       Collapse<double>((double2*)s, n, mask, bits));
+  return GuardHost([&]() -> int {
+    QHIP_TRY(CheckState(prec, s, n));
+    return QHIP_DISPATCH(prec, Collapse<float>((float2*)s, n, mask, bits),
+                         Collapse<double>((double2*)s, n, mask, bits));
+  });
 }
 
 // One gate pass now (no queue).
@@ -657,12 +661,14 @@ int qsim_hip_apply_controlled_gate(int prec, void* s, unsigned n,
                                    const unsigned* cqs, unsigned nc,
                                    uint64_t cvals, const void* matrix,
                                    int matrix_prec) {
-  // Without controls this is a plain gate, which the gate queue may take.
-  if (nc == 0) {
-    return qsim_hip_apply_gate(prec, s, n, qs, k, matrix, matrix_prec);
-  }
-  QHIP_TRY(CheckState(prec, s, n));
-  return ApplyGateNow(prec, s, n, qs, k, cqs, nc, cvals, matrix, matrix_prec);
+  return GuardHost([&]() -> int {
+    // Without controls this is a plain gate, which the gate queue may take.
+    if (nc == 0) {
+      return qsim_hip_apply_gate(prec, s, n, qs, k, matrix, matrix_prec);
+    }
+    QHIP_TRY(CheckState(prec, s, n));
+    return ApplyGateNow(prec, s, n, qs, k, cqs, nc, cvals, matrix, matrix_prec);
+  });
 }
 
 }  // extern "C"
@@ -827,11 +833,13 @@ unsigned qsim_hip_block_resident_ops(unsigned tile_bits) {
 
 int qsim_hip_apply_gate(int prec, void* s, unsigned n, const unsigned* qs,
                         unsigned k, const void* matrix, int matrix_prec) {
-  QHIP_TRY(CheckStateArgs(prec, s, n));
-  bool queued = false;
-  QHIP_TRY(Enqueue(prec, s, n, qs, k, matrix, matrix_prec, &queued));
-  if (queued) return QSIM_HIP_OK;
-  return ApplyGateNow(prec, s, n, qs, k, nullptr, 0, 0, matrix, matrix_prec);
+  return GuardHost([&]() -> int {
+    QHIP_TRY(CheckStateArgs(prec, s, n));
+    bool queued = false;
+    QHIP_TRY(Enqueue(prec, s, n, qs, k, matrix, matrix_prec, &queued));
+    if (queued) return QSIM_HIP_OK;
+    return ApplyGateNow(prec, s, n, qs, k, nullptr, 0, 0, matrix, matrix_prec);
+  });
 }
 
 int qsim_hip_apply_gate_block(int prec, void* s, unsigned n,
@@ -840,32 +848,34 @@ int qsim_hip_apply_gate_block(int prec, void* s, unsigned n,
                               const unsigned* gate_num_qubits,
                               const unsigned* gate_qubits,
                               const void* matrices, int matrix_prec) {
-  QHIP_TRY(CheckState(prec, s, n));
-  if (tile_qubits == nullptr ||
-      (num_gates > 0 && (gate_num_qubits == nullptr ||
-                         gate_qubits == nullptr || matrices == nullptr))) {
-    SetError("null blocked-pass operand");
-    return QSIM_HIP_ERR_ARG;
-  }
-  if (prec != QSIM_HIP_F32) {
-    SetError("blocked passes run on complex64 states only");
-    return QSIM_HIP_ERR_UNSUPPORTED;
-  }
-  if (num_gates == 0) return QSIM_HIP_OK;
-  const bool accurate = (matrix_prec & QSIM_HIP_BLOCK_ACCURATE) != 0;
-  const int arith = (matrix_prec & QSIM_HIP_BLOCK_BF16) != 0
-                        ? int(kBlockArithBf16)
-                        : int(kBlockArithF16);
-  std::vector<size_t> moff;
-  std::vector<double> m64;
-  QHIP_TRY(BlockMatrices(
-      num_gates, gate_num_qubits, matrices,
-      matrix_prec & ~(QSIM_HIP_BLOCK_ACCURATE | QSIM_HIP_BLOCK_BF16), moff,
-      m64));
-  return LaunchGateBlock(static_cast<float2*>(s), n, tile_qubits,
-                         num_tile_qubits, num_gates, gate_num_qubits,
-                         gate_qubits, m64.data(), moff.data(), accurate,
-                         arith);
+  return GuardHost([&]() -> int {
+    QHIP_TRY(CheckState(prec, s, n));
+    if (tile_qubits == nullptr ||
+        (num_gates > 0 && (gate_num_qubits == nullptr ||
+                           gate_qubits == nullptr || matrices == nullptr))) {
+      SetError("null blocked-pass operand");
+      return QSIM_HIP_ERR_ARG;
+    }
+    if (prec != QSIM_HIP_F32) {
+      SetError("blocked passes run on complex64 states only");
+      return QSIM_HIP_ERR_UNSUPPORTED;
+    }
+    if (num_gates == 0) return QSIM_HIP_OK;
+    const bool accurate = (matrix_prec & QSIM_HIP_BLOCK_ACCURATE) != 0;
+    const int arith = (matrix_prec & QSIM_HIP_BLOCK_BF16) != 0
+                          ? int(kBlockArithBf16)
+                          : int(kBlockArithF16);
+    std::vector<size_t> moff;
+    std::vector<double> m64;
+    QHIP_TRY(BlockMatrices(
+        num_gates, gate_num_qubits, matrices,
+        matrix_prec & ~(QSIM_HIP_BLOCK_ACCURATE | QSIM_HIP_BLOCK_BF16), moff,
+        m64));
+    return LaunchGateBlock(static_cast<float2*>(s), n, tile_qubits,
+                           num_tile_qubits, num_gates, gate_num_qubits,
+                           gate_qubits, m64.data(), moff.data(), accurate,
+                           arith);
+  });
 }
 
 int qsim_hip_block_prepare(unsigned n, const unsigned* tile_qubits,
@@ -873,27 +883,29 @@ int qsim_hip_block_prepare(unsigned n, const unsigned* tile_qubits,
                            const unsigned* gate_num_qubits,
                            const unsigned* gate_qubits, const void* matrices,
                            int matrix_prec, void** handle) {
-  if (handle == nullptr || tile_qubits == nullptr || num_gates == 0 ||
-      gate_num_qubits == nullptr || gate_qubits == nullptr ||
-      matrices == nullptr) {
-    SetError("null or empty blocked-pass operand");
-    return QSIM_HIP_ERR_ARG;
-  }
-  *handle = nullptr;
-  QHIP_TRY(EnsureInit());
-  const bool accurate = (matrix_prec & QSIM_HIP_BLOCK_ACCURATE) != 0;
-  const int arith = (matrix_prec & QSIM_HIP_BLOCK_BF16) != 0
-                        ? int(kBlockArithBf16)
-                        : int(kBlockArithF16);
-  std::vector<size_t> moff;
-  std::vector<double> m64;
-  QHIP_TRY(BlockMatrices(
-      num_gates, gate_num_qubits, matrices,
-      matrix_prec & ~(QSIM_HIP_BLOCK_ACCURATE | QSIM_HIP_BLOCK_BF16), moff,
-      m64));
-  return PrepareBlockHandle(n, tile_qubits, num_tile_qubits, num_gates,
-                            gate_num_qubits, gate_qubits, m64.data(),
-                            moff.data(), handle, accurate, arith);
+  return GuardHost([&]() -> int {
+    if (handle == nullptr || tile_qubits == nullptr || num_gates == 0 ||
+        gate_num_qubits == nullptr || gate_qubits == nullptr ||
+        matrices == nullptr) {
+      SetError("null or empty blocked-pass operand");
+      return QSIM_HIP_ERR_ARG;
+    }
+    *handle = nullptr;
+    QHIP_TRY(EnsureInit());
+    const bool accurate = (matrix_prec & QSIM_HIP_BLOCK_ACCURATE) != 0;
+    const int arith = (matrix_prec & QSIM_HIP_BLOCK_BF16) != 0
+                          ? int(kBlockArithBf16)
+                          : int(kBlockArithF16);
+    std::vector<size_t> moff;
+    std::vector<double> m64;
+    QHIP_TRY(BlockMatrices(
+        num_gates, gate_num_qubits, matrices,
+        matrix_prec & ~(QSIM_HIP_BLOCK_ACCURATE | QSIM_HIP_BLOCK_BF16), moff,
+        m64));
+    return PrepareBlockHandle(n, tile_qubits, num_tile_qubits, num_gates,
+                              gate_num_qubits, gate_qubits, m64.data(),
+                              moff.data(), handle, accurate, arith);
+  });
 }
 
 int qsim_hip_block_run(void* handle, int prec, void* s, unsigned n) {
@@ -918,37 +930,39 @@ int qsim_hip_expectation_value(int prec, const void* s, unsigned n,
                                const unsigned* qs, unsigned k,
                                const void* matrix, int matrix_prec, double* re,
                                double* im) {
-  QHIP_TRY(CheckState(prec, s, n));
-  if (qs == nullptr || matrix == nullptr || re == nullptr || im == nullptr) {
-    SetError("null operand");
-    return QSIM_HIP_ERR_ARG;
-  }
-  if (k > 6) {
-    SetError("operators on %u > 6 qubits are not supported", k);
-    return QSIM_HIP_ERR_UNSUPPORTED;
-  }
-  const size_t melems = size_t{2} << (2 * k);
-  if (prec == QSIM_HIP_F32) {
-    std::vector<float> mf;
-    const float* m = static_cast<const float*>(matrix);
-    if (matrix_prec == QSIM_HIP_F64) {
-      mf.resize(melems);
-      for (size_t i = 0; i < melems; ++i)
-        mf[i] = float(static_cast<const double*>(matrix)[i]);
-      m = mf.data();
+  return GuardHost([&]() -> int {
+    QHIP_TRY(CheckState(prec, s, n));
+    if (qs == nullptr || matrix == nullptr || re == nullptr || im == nullptr) {
+      SetError("null operand");
+      return QSIM_HIP_ERR_ARG;
     }
-    return LaunchExpectation<float>((const float2*)s, n, qs, k, m, re, im);
-  } else {
-    std::vector<double> md;
-    const double* m = static_cast<const double*>(matrix);
-    if (matrix_prec == QSIM_HIP_F32) {
-      md.resize(melems);
-      for (size_t i = 0; i < melems; ++i)
-        md[i] = static_cast<const float*>(matrix)[i];
-      m = md.data();
+    if (k > 6) {
+      SetError("operators on %u > 6 qubits are not supported", k);
+      return QSIM_HIP_ERR_UNSUPPORTED;
     }
-    return LaunchExpectation<double>((const double2*)s, n, qs, k, m, re, im);
-  }
+    const size_t melems = size_t{2} << (2 * k);
+    if (prec == QSIM_HIP_F32) {
+      std::vector<float> mf;
+      const float* m = static_cast<const float*>(matrix);
+      if (matrix_prec == QSIM_HIP_F64) {
+        mf.resize(melems);
+        for (size_t i = 0; i < melems; ++i)
+          mf[i] = float(static_cast<const double*>(matrix)[i]);
+        m = mf.data();
+      }
+      return LaunchExpectation<float>((const float2*)s, n, qs, k, m, re, im);
+    } else {
+      std::vector<double> md;
+      const double* m = static_cast<const double*>(matrix);
+      if (matrix_prec == QSIM_HIP_F32) {
+        md.resize(melems);
+        for (size_t i = 0; i < melems; ++i)
+          md[i] = static_cast<const float*>(matrix)[i];
+        m = md.data();
+      }
+      return LaunchExpectation<double>((const double2*)s, n, qs, k, m, re, im);
+    }
+  });
 }
 
 int qsim_hip_set_gate_method(int method) {
@@ -985,20 +999,22 @@ int qsim_hip_event_elapsed_ms(int a, int b, float* ms) {
 }
 
 int qsim_hip_graph_record_begin(void** plan) {
-  QHIP_TRY(EnsureInit());
-  QHIP_TRY(FlushGateQueue());
-  if (plan == nullptr) {
-    SetError("null plan handle");
-    return QSIM_HIP_ERR_ARG;
-  }
-  if (g_plan_mode != kPlanOff) {
-    SetError("a pass sequence is already being recorded or captured");
-    return QSIM_HIP_ERR_ARG;
-  }
-  g_plan = new GraphPlan{};
-  g_plan_mode = kPlanRecord;
-  *plan = g_plan;
-  return QSIM_HIP_OK;
+  return GuardHost([&]() -> int {
+    QHIP_TRY(EnsureInit());
+    QHIP_TRY(FlushGateQueue());
+    if (plan == nullptr) {
+      SetError("null plan handle");
+      return QSIM_HIP_ERR_ARG;
+    }
+    if (g_plan_mode != kPlanOff) {
+      SetError("a pass sequence is already being recorded or captured");
+      return QSIM_HIP_ERR_ARG;
+    }
+    g_plan = new GraphPlan{};
+    g_plan_mode = kPlanRecord;
+    *plan = g_plan;
+    return QSIM_HIP_OK;
+  });
 }
 
 int qsim_hip_graph_record_end(void* plan) {
@@ -1084,21 +1100,23 @@ int qsim_hip_graph_free(void* plan) {
 }
 
 int qsim_hip_pass_timer_start(unsigned capacity) {
-  QHIP_TRY(EnsureInit());
-  QHIP_TRY(FlushGateQueue());
-  if (capacity == 0 || capacity > (1u << 20)) {
-    SetError("pass timer capacity %u out of range", capacity);
-    return QSIM_HIP_ERR_ARG;
-  }
-  while (g_timer.ev.size() < 2 * size_t(capacity)) {
-    hipEvent_t e;
-    QHIP_CHECK(hipEventCreate(&e));
-    g_timer.ev.push_back(e);
-  }
-  g_timer.capacity = capacity;
-  g_timer.count = 0;
-  g_timer.on = true;
-  return QSIM_HIP_OK;
+  return GuardHost([&]() -> int {
+    QHIP_TRY(EnsureInit());
+    QHIP_TRY(FlushGateQueue());
+    if (capacity == 0 || capacity > (1u << 20)) {
+      SetError("pass timer capacity %u out of range", capacity);
+      return QSIM_HIP_ERR_ARG;
+    }
+    while (g_timer.ev.size() < 2 * size_t(capacity)) {
+      hipEvent_t e;
+      QHIP_CHECK(hipEventCreate(&e));
+      g_timer.ev.push_back(e);
+    }
+    g_timer.capacity = capacity;
+    g_timer.count = 0;
+    g_timer.on = true;
+    return QSIM_HIP_OK;
+  });
 }
 
 int qsim_hip_pass_timer_stop(float* ms, unsigned max_count, unsigned* count) {

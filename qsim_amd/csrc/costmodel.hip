@@ -256,58 +256,60 @@ extern "C" {
 
 int qsim_hip_costmodel_create(unsigned num_features, unsigned hidden,
                               void** model) {
-  if (model == nullptr || num_features == 0 || num_features > 256 ||
-      hidden == 0 || hidden > 256) {
-    SetError("cost model: bad sizes (features %u, hidden %u)", num_features,
-             hidden);
-    return QSIM_HIP_ERR_ARG;
-  }
-  *model = nullptr;
-  QHIP_TRY(EnsureInit());
-  auto* c = new CostModel;
-  c->F = num_features;
-  c->H = hidden;
-  size_t off = 0;
-  for (int l = 0; l < kLayers; ++l) {
-    c->in[l] = l == 0 ? num_features : hidden;
-    c->out[l] = l == kLayers - 1 ? 1 : hidden;
-    c->woff[l] = off;
-    off += size_t(c->in[l]) * c->out[l];
-    c->boff[l] = off;
-    off += c->out[l];
-  }
-  c->nparams = off;
-  const size_t bytes = off * sizeof(float);
-  hipError_t e = hipMalloc(&c->params, bytes);
-  if (e == hipSuccess) e = hipMalloc(&c->grads, bytes);
-  if (e == hipSuccess) e = hipMalloc(&c->m, bytes);
-  if (e == hipSuccess) e = hipMalloc(&c->v, bytes);
-  if (e != hipSuccess) {
-    (void)hipFree(c->params);
-    (void)hipFree(c->grads);
-    (void)hipFree(c->m);
-    (void)hipFree(c->v);
-    delete c;
-    return HipFail(e, "hipMalloc(cost model)");
-  }
-  // He-style initialisation from a fixed seed; biases zero.
-  std::vector<float> p(off, 0.0f);
-  std::mt19937 rng(12345);
-  for (int l = 0; l < kLayers; ++l) {
-    std::normal_distribution<float> d(0.0f, std::sqrt(2.0f / c->in[l]));
-    for (size_t i = 0; i < size_t(c->in[l]) * c->out[l]; ++i)
-      p[c->woff[l] + i] = d(rng);
-  }
-  QHIP_CHECK(hipMemcpyAsync(c->params, p.data(), bytes, hipMemcpyHostToDevice,
-                            Stream()));
-  hipLaunchKernelGGL(FillZero, dim3(Blocks(off)), dim3(256), 0, Stream(), c->m,
-                     unsigned(off));
-  hipLaunchKernelGGL(FillZero, dim3(Blocks(off)), dim3(256), 0, Stream(), c->v,
-                     unsigned(off));
-  QHIP_LAUNCHED("FillZero");
-  QHIP_CHECK(hipStreamSynchronize(Stream()));
-  *model = c;
-  return QSIM_HIP_OK;
+  return GuardHost([&]() -> int {
+    if (model == nullptr || num_features == 0 || num_features > 256 ||
+        hidden == 0 || hidden > 256) {
+      SetError("cost model: bad sizes (features %u, hidden %u)", num_features,
+               hidden);
+      return QSIM_HIP_ERR_ARG;
+    }
+    *model = nullptr;
+    QHIP_TRY(EnsureInit());
+    auto* c = new CostModel;
+    c->F = num_features;
+    c->H = hidden;
+    size_t off = 0;
+    for (int l = 0; l < kLayers; ++l) {
+      c->in[l] = l == 0 ? num_features : hidden;
+      c->out[l] = l == kLayers - 1 ? 1 : hidden;
+      c->woff[l] = off;
+      off += size_t(c->in[l]) * c->out[l];
+      c->boff[l] = off;
+      off += c->out[l];
+    }
+    c->nparams = off;
+    const size_t bytes = off * sizeof(float);
+    hipError_t e = hipMalloc(&c->params, bytes);
+    if (e == hipSuccess) e = hipMalloc(&c->grads, bytes);
+    if (e == hipSuccess) e = hipMalloc(&c->m, bytes);
+    if (e == hipSuccess) e = hipMalloc(&c->v, bytes);
+    if (e != hipSuccess) {
+      (void)hipFree(c->params);
+      (void)hipFree(c->grads);
+      (void)hipFree(c->m);
+      (void)hipFree(c->v);
+      delete c;
+      return HipFail(e, "hipMalloc(cost model)");
+    }
+    // He-style initialisation from a fixed seed; biases zero.
+    std::vector<float> p(off, 0.0f);
+    std::mt19937 rng(12345);
+    for (int l = 0; l < kLayers; ++l) {
+      std::normal_distribution<float> d(0.0f, std::sqrt(2.0f / c->in[l]));
+      for (size_t i = 0; i < size_t(c->in[l]) * c->out[l]; ++i)
+        p[c->woff[l] + i] = d(rng);
+    }
+    QHIP_CHECK(hipMemcpyAsync(c->params, p.data(), bytes, hipMemcpyHostToDevice,
+                              Stream()));
+    hipLaunchKernelGGL(FillZero, dim3(Blocks(off)), dim3(256), 0, Stream(), c->m,
+                       unsigned(off));
+    hipLaunchKernelGGL(FillZero, dim3(Blocks(off)), dim3(256), 0, Stream(), c->v,
+                       unsigned(off));
+    QHIP_LAUNCHED("FillZero");
+    QHIP_CHECK(hipStreamSynchronize(Stream()));
+    *model = c;
+    return QSIM_HIP_OK;
+  });
 }
 
 // The process's default model (qsim_hip_costmodel_set_default).
@@ -321,62 +323,64 @@ int qsim_hip_costmodel_set_default(void* model) {
 }
 
 int qsim_hip_costmodel_load(const char* path, void** model) {
-  if (path == nullptr || model == nullptr) {
-    SetError("cost model: null path or handle");
-    return QSIM_HIP_ERR_ARG;
-  }
-  *model = nullptr;
-  std::FILE* f = std::fopen(path, "rb");
-  if (f == nullptr) {
-    SetError("cost model: cannot open %s", path);
-    return QSIM_HIP_ERR_ARG;
-  }
-  std::string text;
-  char buf[65536];
-  size_t got;
-  while ((got = std::fread(buf, 1, sizeof(buf), f)) > 0) text.append(buf, got);
-  std::fclose(f);
-  // {"num_features": F, "hidden": H, "params": [p0, p1, ...], ...}
-  auto number_after = [&](const char* key, double* v) {
-    const size_t k = text.find(key);
-    if (k == std::string::npos) return false;
-    const size_t c = text.find(':', k);
-    if (c == std::string::npos) return false;
-    char* end = nullptr;
-    *v = std::strtod(text.c_str() + c + 1, &end);
-    return end != text.c_str() + c + 1;
-  };
-  double F = 0, H = 0;
-  const size_t pk = text.find("\"params\"");
-  const size_t lb = pk == std::string::npos ? pk : text.find('[', pk);
-  if (!number_after("\"num_features\"", &F) ||
-      !number_after("\"hidden\"", &H) || lb == std::string::npos) {
-    SetError("cost model: %s lacks num_features, hidden or params", path);
-    return QSIM_HIP_ERR_ARG;
-  }
-  std::vector<float> params;
-  const char* p = text.c_str() + lb + 1;
-  while (true) {
-    while (*p == ' ' || *p == '\n' || *p == '\r' || *p == '\t' || *p == ',') ++p;
-    if (*p == ']' || *p == 0) break;
-    char* end = nullptr;
-    const double v = std::strtod(p, &end);
-    if (end == p) {
-      SetError("cost model: bad number in %s", path);
+  return GuardHost([&]() -> int {
+    if (path == nullptr || model == nullptr) {
+      SetError("cost model: null path or handle");
       return QSIM_HIP_ERR_ARG;
     }
-    params.push_back(float(v));
-    p = end;
-  }
-  void* m = nullptr;
-  QHIP_TRY(qsim_hip_costmodel_create(unsigned(F), unsigned(H), &m));
-  const int rc = qsim_hip_costmodel_set_params(m, params.data(), params.size());
-  if (rc != QSIM_HIP_OK) {
-    qsim_hip_costmodel_free(m);
-    return rc;
-  }
-  *model = m;
-  return QSIM_HIP_OK;
+    *model = nullptr;
+    std::FILE* f = std::fopen(path, "rb");
+    if (f == nullptr) {
+      SetError("cost model: cannot open %s", path);
+      return QSIM_HIP_ERR_ARG;
+    }
+    std::string text;
+    char buf[65536];
+    size_t got;
+    while ((got = std::fread(buf, 1, sizeof(buf), f)) > 0) text.append(buf, got);
+    std::fclose(f);
+    // {"num_features": F, "hidden": H, "params": [p0, p1, ...], ...}
+    auto number_after = [&](const char* key, double* v) {
+      const size_t k = text.find(key);
+      if (k == std::string::npos) return false;
+      const size_t c = text.find(':', k);
+      if (c == std::string::npos) return false;
+      char* end = nullptr;
+      *v = std::strtod(text.c_str() + c + 1, &end);
+      return end != text.c_str() + c + 1;
+    };
+    double F = 0, H = 0;
+    const size_t pk = text.find("\"params\"");
+    const size_t lb = pk == std::string::npos ? pk : text.find('[', pk);
+    if (!number_after("\"num_features\"", &F) ||
+        !number_after("\"hidden\"", &H) || lb == std::string::npos) {
+      SetError("cost model: %s lacks num_features, hidden or params", path);
+      return QSIM_HIP_ERR_ARG;
+    }
+    std::vector<float> params;
+    const char* p = text.c_str() + lb + 1;
+    while (true) {
+      while (*p == ' ' || *p == '\n' || *p == '\r' || *p == '\t' || *p == ',') ++p;
+      if (*p == ']' || *p == 0) break;
+      char* end = nullptr;
+      const double v = std::strtod(p, &end);
+      if (end == p) {
+        SetError("cost model: bad number in %s", path);
+        return QSIM_HIP_ERR_ARG;
+      }
+      params.push_back(float(v));
+      p = end;
+    }
+    void* m = nullptr;
+    QHIP_TRY(qsim_hip_costmodel_create(unsigned(F), unsigned(H), &m));
+    const int rc = qsim_hip_costmodel_set_params(m, params.data(), params.size());
+    if (rc != QSIM_HIP_OK) {
+      qsim_hip_costmodel_free(m);
+      return rc;
+    }
+    *model = m;
+    return QSIM_HIP_OK;
+  });
 }
 
 int qsim_hip_costmodel_free(void* model) {
@@ -478,90 +482,92 @@ int qsim_hip_costmodel_train(void* model, const float* x, const float* t,
                              unsigned num_samples, unsigned epochs,
                              unsigned batch, float lr, unsigned seed,
                              double* final_loss) {
-  auto* c = static_cast<CostModel*>(model);
-  if (c == nullptr || x == nullptr || t == nullptr || num_samples == 0 ||
-      batch == 0) {
-    SetError("cost model: bad training arguments");
-    return QSIM_HIP_ERR_ARG;
-  }
-  batch = std::min(batch, num_samples);
-  QHIP_TRY(Reserve(*c, std::max(batch, num_samples)));
-  // The training set stays on the device; each epoch uploads one seeded
-  // permutation and every step gathers its minibatch from it.
-  float *xall = nullptr, *tall = nullptr;
-  unsigned* idx = nullptr;
-  auto release = [&]() {
-    (void)hipStreamSynchronize(Stream());
-    (void)hipFree(xall);
-    (void)hipFree(tall);
-    (void)hipFree(idx);
-  };
-  hipError_t e = hipMalloc(&xall, size_t(num_samples) * c->F * sizeof(float));
-  if (e == hipSuccess) e = hipMalloc(&tall, size_t(num_samples) * sizeof(float));
-  if (e == hipSuccess) e = hipMalloc(&idx, size_t(num_samples) * sizeof(unsigned));
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(xall, x, size_t(num_samples) * c->F * sizeof(float),
-                       hipMemcpyHostToDevice, Stream());
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(tall, t, size_t(num_samples) * sizeof(float),
-                       hipMemcpyHostToDevice, Stream());
-  if (e != hipSuccess) {
-    release();
-    return HipFail(e, "cost model training buffers");
-  }
-  std::vector<unsigned> order(num_samples);
-  for (unsigned i = 0; i < num_samples; ++i) order[i] = i;
-  std::mt19937 rng(seed);
-  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-  for (unsigned ep = 0; ep < epochs; ++ep) {
-    std::shuffle(order.begin(), order.end(), rng);
-    e = hipMemcpyAsync(idx, order.data(), size_t(num_samples) * sizeof(unsigned),
-                       hipMemcpyHostToDevice, Stream());
+  return GuardHost([&]() -> int {
+    auto* c = static_cast<CostModel*>(model);
+    if (c == nullptr || x == nullptr || t == nullptr || num_samples == 0 ||
+        batch == 0) {
+      SetError("cost model: bad training arguments");
+      return QSIM_HIP_ERR_ARG;
+    }
+    batch = std::min(batch, num_samples);
+    QHIP_TRY(Reserve(*c, std::max(batch, num_samples)));
+    // The training set stays on the device; each epoch uploads one seeded
+    // permutation and every step gathers its minibatch from it.
+    float *xall = nullptr, *tall = nullptr;
+    unsigned* idx = nullptr;
+    auto release = [&]() {
+      (void)hipStreamSynchronize(Stream());
+      (void)hipFree(xall);
+      (void)hipFree(tall);
+      (void)hipFree(idx);
+    };
+    hipError_t e = hipMalloc(&xall, size_t(num_samples) * c->F * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&tall, size_t(num_samples) * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&idx, size_t(num_samples) * sizeof(unsigned));
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(xall, x, size_t(num_samples) * c->F * sizeof(float),
+                         hipMemcpyHostToDevice, Stream());
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(tall, t, size_t(num_samples) * sizeof(float),
+                         hipMemcpyHostToDevice, Stream());
     if (e != hipSuccess) {
       release();
-      return HipFail(e, "cost model permutation");
+      return HipFail(e, "cost model training buffers");
     }
-    for (unsigned s0 = 0; s0 + batch <= num_samples; s0 += batch) {
-      hipLaunchKernelGGL(Gather, dim3(Blocks(size_t(batch) * c->F)), dim3(256),
-                         0, Stream(), xall, tall, idx + s0, batch, c->F, c->x,
-                         c->t);
-      int rc = Forward(*c, batch);
-      if (rc == QSIM_HIP_OK) rc = Backward(*c, batch);
-      if (rc != QSIM_HIP_OK) {
+    std::vector<unsigned> order(num_samples);
+    for (unsigned i = 0; i < num_samples; ++i) order[i] = i;
+    std::mt19937 rng(seed);
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    for (unsigned ep = 0; ep < epochs; ++ep) {
+      std::shuffle(order.begin(), order.end(), rng);
+      e = hipMemcpyAsync(idx, order.data(), size_t(num_samples) * sizeof(unsigned),
+                         hipMemcpyHostToDevice, Stream());
+      if (e != hipSuccess) {
         release();
-        return rc;
+        return HipFail(e, "cost model permutation");
       }
-      ++c->adam_t;
-      const float c1 = 1.0f - std::pow(b1, float(c->adam_t));
-      const float c2 = 1.0f - std::pow(b2, float(c->adam_t));
-      hipLaunchKernelGGL(AdamStep, dim3(Blocks(c->nparams)), dim3(256), 0,
-                         Stream(), c->params, c->grads, c->m, c->v,
-                         unsigned(c->nparams), lr, b1, b2, eps, c1, c2);
+      for (unsigned s0 = 0; s0 + batch <= num_samples; s0 += batch) {
+        hipLaunchKernelGGL(Gather, dim3(Blocks(size_t(batch) * c->F)), dim3(256),
+                           0, Stream(), xall, tall, idx + s0, batch, c->F, c->x,
+                           c->t);
+        int rc = Forward(*c, batch);
+        if (rc == QSIM_HIP_OK) rc = Backward(*c, batch);
+        if (rc != QSIM_HIP_OK) {
+          release();
+          return rc;
+        }
+        ++c->adam_t;
+        const float c1 = 1.0f - std::pow(b1, float(c->adam_t));
+        const float c2 = 1.0f - std::pow(b2, float(c->adam_t));
+        hipLaunchKernelGGL(AdamStep, dim3(Blocks(c->nparams)), dim3(256), 0,
+                           Stream(), c->params, c->grads, c->m, c->v,
+                           unsigned(c->nparams), lr, b1, b2, eps, c1, c2);
+      }
+      // `order` is rewritten next epoch: its upload must have finished.
+      e = hipStreamSynchronize(Stream());
+      if (e != hipSuccess) {
+        release();
+        return HipFail(e, "cost model epoch");
+      }
     }
-    // `order` is rewritten next epoch: its upload must have finished.
-    e = hipStreamSynchronize(Stream());
-    if (e != hipSuccess) {
-      release();
-      return HipFail(e, "cost model epoch");
-    }
-  }
-  // Mean squared error over the whole training set.
-  e = hipMemcpyAsync(c->x, xall, size_t(num_samples) * c->F * sizeof(float),
-                     hipMemcpyDeviceToDevice, Stream());
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(c->t, tall, size_t(num_samples) * sizeof(float),
+    // Mean squared error over the whole training set.
+    e = hipMemcpyAsync(c->x, xall, size_t(num_samples) * c->F * sizeof(float),
                        hipMemcpyDeviceToDevice, Stream());
-  release();
-  if (e != hipSuccess) return HipFail(e, "cost model evaluation copy");
-  QHIP_TRY(Forward(*c, num_samples));
-  hipLaunchKernelGGL(MseGrad, dim3(Blocks(num_samples)), dim3(256), 0, Stream(),
-                     c->z[kLayers - 1], c->t, c->dz[kLayers - 1], c->loss,
-                     num_samples);
-  QHIP_LAUNCHED("MseGrad");
-  double l = 0;
-  QHIP_TRY(LossSum(*c, num_samples, &l));
-  if (final_loss) *final_loss = l;
-  return QSIM_HIP_OK;
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(c->t, tall, size_t(num_samples) * sizeof(float),
+                         hipMemcpyDeviceToDevice, Stream());
+    release();
+    if (e != hipSuccess) return HipFail(e, "cost model evaluation copy");
+    QHIP_TRY(Forward(*c, num_samples));
+    hipLaunchKernelGGL(MseGrad, dim3(Blocks(num_samples)), dim3(256), 0, Stream(),
+                       c->z[kLayers - 1], c->t, c->dz[kLayers - 1], c->loss,
+                       num_samples);
+    QHIP_LAUNCHED("MseGrad");
+    double l = 0;
+    QHIP_TRY(LossSum(*c, num_samples, &l));
+    if (final_loss) *final_loss = l;
+    return QSIM_HIP_OK;
+  });
 }
 
 }  // extern "C"

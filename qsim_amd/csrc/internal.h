@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <mutex>
+#include <new>
 #include <vector>
 
 #include "qsim_hip.h"
@@ -14,6 +15,18 @@ namespace qhip {
 // ---- error reporting -------------------------------------------------------
 
 void SetError(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
+
+// Runs an entry point's body: a failed host allocation becomes an error
+// return instead of an exception crossing the C boundary.
+template <typename F>
+int GuardHost(F&& f) {
+  try {
+    return f();
+  } catch (const std::bad_alloc&) {
+    SetError("host memory allocation failed");
+    return QSIM_HIP_ERR_RUNTIME;
+  }
+}
 int HipFail(hipError_t e, const char* what);
 
 #define QHIP_CHECK(expr)                                   \
