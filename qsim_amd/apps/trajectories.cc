@@ -437,6 +437,8 @@ bool FlushTiled(const std::vector<Operation<double>>& deferred, unsigned n,
   hip::BlockPlannerParameter bp;
   bp.tile_bits = spec.tile_bits;
   bp.low_bits = spec.low_bits;
+  // No more fused gates per pass than the tile kernel keeps resident.
+  bp.max_ops = qsim_hip_block_resident_ops(std::min(spec.tile_bits, n));
   hip::BlockPlan blocks;
   const std::function<void(FusedGate<double>&)> fill =
       [&cache](FusedGate<double>& fg) { cache.Fill(fg); };

@@ -33,6 +33,7 @@
 #include <memory>
 #include <atomic>
 #include <cmath>
+#include <cstring>
 #include <mutex>
 #include <new>
 #include <string>
@@ -408,7 +409,7 @@ TrajectoryKernel(const qsim_hip_traj_desc* __restrict__ descs,
 }
 
 // A tile pass of a trajectory on the device: geometry, program, and the
-// index of its first op record (kOpWords words each) in the record pool.
+// index of its first op record (kF16RecWords words each) in the record pool.
 struct TrajTilePass {
   BlockGeom g;
   BlockProgram p;
@@ -419,126 +420,84 @@ struct TrajTilePass {
 // qubits, 256 threads per trajectory, two workgroups per CU; TB = 14, tiles
 // of up to 14 qubits, 512 threads, one workgroup per CU (twice the bytes in
 // flight per trajectory, fewer passes, half as many trajectories at once).
-// One tile pass of the workgroup's trajectory: for every tile of the state,
-// stage it in LDS, run the pass's ops on it (split-BF16 MFMA with the
-// amplitudes in three parts), write it
-// back, with the next tile's loads in flight meanwhile.  The pass's op
-// records cycle through a two-slot ring in LDS as in the circuit kernel's
-// blocked passes (block_kernels.hip).  The tile sits at LDS offset 0 (no
-// static LDS in the kernel, checked on the host).  FULL: tiles of TB
-// qubits, whose next-tile loads go out in quarters over the first ops (each
-// quarter right after an op's record loads, so that in-order completion
-// never makes a record wait behind it); smaller tiles issue them all during
-// the last op.
+// One tile pass of the workgroup's trajectory, with the FP16 arithmetic of
+// the circuit kernel's blocked passes (block_kernels.hip): the pass's op
+// records are copied into LDS once, then for every tile of the state the
+// tile is staged in LDS scaled by a power of two (TileScale), the ops run
+// on it (FP16 MFMA, two parts per operand), and it is stored back scaled
+// the other way, with the next tile's loads in flight meanwhile (in two
+// halves around the first ops).  A pass's ops are unitary or Kraus
+// operators (spectral norm <= 1), so the tile never grows.  The tile sits at
+// LDS offset 0 (no static LDS in the kernel, checked on the host).  FULL:
+// tiles of TB qubits.
 template <int TB, bool FULL>
 __device__ __forceinline__ void TrajTileRun(float2* __restrict__ s,
                                             const TrajTilePass* __restrict__ tp,
-                                            const uint16_t* __restrict__ recs,
+                                            const uint32_t* __restrict__ recs,
                                             char* lds_raw) {
   constexpr int NT = BlockShape<TB>::NT;
+  constexpr int NW = BlockShape<TB>::kWaves;
   constexpr int PER = 32;
-  constexpr int kRingQuads = kOpWords / 4 / NT;
-  static_assert(kOpWords % (4 * NT) == 0 && (1 << TB) / NT == PER, "shape");
+  static_assert((1 << TB) / NT == PER, "shape");
   float2* lds = reinterpret_cast<float2*>(lds_raw);
-  uint32_t* ring =
-      reinterpret_cast<uint32_t*>(lds_raw + BlockShape<TB>::kTileBytes);
-  uint4* ring4 = reinterpret_cast<uint4*>(ring);
+  float* slots = reinterpret_cast<float*>(lds_raw + F16Lds<TB>::kMaxOff);
+  uint32_t* rec = reinterpret_cast<uint32_t*>(lds_raw + F16Lds<TB>::kRecOff);
   const BlockGeom& g = tp->g;
   const BlockProgram& p = tp->p;
   const unsigned nops = p.nops;
   const int l = threadIdx.x & 63;
   const unsigned wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const unsigned nbatch = (1u << g.tb) >> 9;
-  const uint4* gfrag =
-      reinterpret_cast<const uint4*>(recs) + tp->rec * (kOpWords / 4);
-  __syncthreads();  // the ring area was scratch until now
-#pragma unroll
-  for (int w = 0; w < kRingQuads; ++w) {
-    ring4[threadIdx.x + NT * w] = gfrag[threadIdx.x + NT * w];
+  const unsigned nper = (1u << g.tb) / NT;
+  __syncthreads();  // the record area was scratch until now
+  {
+    const unsigned nq = nops * unsigned(kF16RecWords / 4);
+    const uint4* src =
+        reinterpret_cast<const uint4*>(recs + tp->rec * kF16RecWords);
+    uint4* dst = reinterpret_cast<uint4*>(rec);
+    for (unsigned q = threadIdx.x; q < nq; q += NT) dst[q] = src[q];
   }
   TileIo<PER, NT> io;
   io.Init(g);
-  unsigned k = 0;
-  auto run_op = [&](unsigned op, unsigned nxt, auto&& extra) {
-    const uint4* gn = gfrag + size_t(nxt) * (kOpWords / 4);
-    const uint4 pre0 = gn[threadIdx.x];
-    const uint4 pre1 = kRingQuads > 1 ? gn[threadIdx.x + NT] : uint4{};
-    extra();
-    const uint32_t* cur = ring + (k & 1u) * kOpWords;
-    const v8bf* fp = reinterpret_cast<const v8bf*>(cur);
-    v8bf f[6];
-#pragma unroll
-    for (int r = 0; r < 6; ++r) f[r] = fp[r * 64 + l];
-    const uint32_t* tab = cur + kFragWords + 16 * (l >> 5);
-    const uint32_t sj = cur[kFragWords + kTableWords + (l & 31)];
-    uint32_t rd[8], wr[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      rd[q] = tab[q] ^ sj;
-      wr[q] = tab[8 + q] ^ sj;
-    }
-    // Amplitudes in three bf16 parts (12 products per 32 groups): a
-    // trajectory's state stays within 1e-5 of FP32 arithmetic.
-    ApplyTileOp<true>(reinterpret_cast<const uint8_t*>(cur + kBposWord), f,
-                      rd, wr, nbatch, wave, BlockShape<TB>::kWaves);
-    uint4* slot = ring4 + ((k + 1) & 1u) * (kOpWords / 4);
-    slot[threadIdx.x] = pre0;
-    if (kRingQuads > 1) slot[threadIdx.x + NT] = pre1;
-    __syncthreads();
-    ++k;
-  };
   float2 buf[PER];
   io.template Fetch<FULL, 15u>(s, 0, g, buf);  // tile 0 starts at amplitude 0
-  io.template Put<FULL>(g, buf, lds);
+  int e = TileScale<NW>(LocalMax<PER, FULL>(buf, nper), slots, p.headroom);
+  io.template PutScaled<FULL>(g, buf, lds, e);
   __syncthreads();
   for (uint64_t tile = 0; tile < g.ntiles; ++tile) {
     const uint64_t base = InsertZeros(tile << g.Lc, g.outer, g.nouter);
     // The next tile, or this one again after the last (an in-bounds read
     // that is never used).
     const uint64_t next = tile + 1 < g.ntiles ? tile + 1 : tile;
-    const uint64_t nbase = InsertZeros(next << g.Lc, g.outer, g.nouter);
-    auto fetch = [&](auto mask) {
-      return [&, mask] {
-        io.template Fetch<FULL, decltype(mask)::value>(s, nbase, g, buf);
-      };
-    };
-    using Q0 = std::integral_constant<unsigned, 1u>;
-    using Q1 = std::integral_constant<unsigned, 2u>;
-    using Q2 = std::integral_constant<unsigned, 4u>;
-    switch (FULL ? nops : 1u) {
-      case 1:
-        for (unsigned op = 0; op + 1 < nops; ++op) run_op(op, op + 1, [] {});
-        run_op(nops - 1, 0, fetch(std::integral_constant<unsigned, 15u>{}));
-        break;
-      case 2:
-        run_op(0, 1, fetch(Q0{}));
-        run_op(1, 0, fetch(std::integral_constant<unsigned, 14u>{}));
-        break;
-      case 3:
-        run_op(0, 1, fetch(Q0{}));
-        run_op(1, 2, fetch(Q1{}));
-        run_op(2, 0, fetch(std::integral_constant<unsigned, 12u>{}));
-        break;
-      default:
-        run_op(0, 1, fetch(Q0{}));
-        run_op(1, 2, fetch(Q1{}));
-        run_op(2, 3, fetch(Q2{}));
-        for (unsigned op = 3; op + 1 < nops; ++op) run_op(op, op + 1, [] {});
-        run_op(nops - 1, 0, fetch(std::integral_constant<unsigned, 8u>{}));
-        break;
+    const uint64_t nb = InsertZeros(next << g.Lc, g.outer, g.nouter);
+    io.template Fetch<FULL, 3u>(s, nb, g, buf);
+    for (unsigned op = 0; op < nops; ++op) {
+      const uint32_t* r = rec + op * unsigned(kF16RecWords);
+      v8h f[4];
+      LoadFragF16(r, l, f);
+      const uint32_t* tab = r + kF16TabOff + 16 * (l >> 5);
+      const uint32_t sj = r[kF16TabOff + kTableWords + (l & 31)];
+      uint32_t mo[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) mo[q] = tab[q] ^ sj;
+      ApplyTileOpF16(reinterpret_cast<const uint8_t*>(r + kF16BposOff), f,
+                     mo, nbatch, wave, NW);
+      __syncthreads();
+      if (op == 1) io.template Fetch<FULL, 12u>(s, nb, g, buf);
     }
-    io.template Store<16, FULL>(s, base, g, lds);
-    __syncthreads();
-    io.template Put<FULL>(g, buf, lds);
+    if (nops == 1) io.template Fetch<FULL, 12u>(s, nb, g, buf);
+    io.template StoreScaled<16, FULL>(s, base, g, lds, e);
+    e = TileScale<NW>(LocalMax<PER, FULL>(buf, nper), slots, p.headroom);
+    io.template PutScaled<FULL>(g, buf, lds, e);
     __syncthreads();
   }
 }
 
 // Workgroup t runs trajectory t with its gate runs as tile passes: one
 // stream of the state per pass instead of one per fused gate.  Gates outside
-// a pass (states under 9 qubits) and damping decisions run as sweeps over
+// a pass (states under 9 qubits) and channel decisions run as sweeps over
 // the state with the workgroup's threads; their scratch (the gate matrix,
-// the reduction partials) reuses the ring area of the LDS.
+// the reduction partials) reuses the record area of the LDS.
 template <int TB>
 __global__ void __launch_bounds__(BlockShape<TB>::NT)
 __attribute__((amdgpu_waves_per_eu(2)))
@@ -546,18 +505,18 @@ TrajTileKernel(const qsim_hip_traj_desc* __restrict__ descs,
                const qsim_hip_traj_op* __restrict__ ops,
                const float* __restrict__ mats,
                const TrajTilePass* __restrict__ passes,
-               const uint16_t* __restrict__ recs,
+               const uint32_t* __restrict__ recs,
                const uint32_t* __restrict__ pairs, float2* __restrict__ pool,
                double* __restrict__ zz_out, double* __restrict__ norm_out,
                int32_t* __restrict__ dec_out) {
   constexpr int NT = BlockShape<TB>::NT;
   extern __shared__ char lds_raw[];
-  // The ring area holds the channel and gate scratch between tile passes.
-  char* aux = lds_raw + BlockShape<TB>::kTileBytes;
+  // The record area holds the channel and gate scratch between tile passes.
+  char* aux = lds_raw + F16Lds<TB>::kRecOff;
   constexpr size_t kLmBytes = 2 * 16 * 16 * sizeof(float);
   static_assert(kLmBytes + (kRedWords<NT> + 2) * sizeof(double) <=
-                    2 * kOpWords * sizeof(uint32_t),
-                "scratch fits the ring area");
+                    F16Lds<TB>::Bytes(F16Lds<TB>::kMaxOps) - F16Lds<TB>::kRecOff,
+                "scratch fits the record area");
   float* lm = reinterpret_cast<float*>(aux);
   double* red = reinterpret_cast<double*>(aux + kLmBytes);
   double* vals = red + kRedWords<NT>;
@@ -585,6 +544,13 @@ TrajTileKernel(const qsim_hip_traj_desc* __restrict__ descs,
   TrajFinish<NT>(s, d, pairs, zz_out, norm_out, red);
 }
 
+// Dynamic LDS of the tiled kernel: the tile, the scale slots and the
+// records of the longest pass its shape keeps resident.
+template <int TB>
+constexpr size_t TrajLds() {
+  return F16Lds<TB>::Bytes(F16Lds<TB>::kMaxOps);
+}
+
 struct TrajBatch {
   unsigned num_traj = 0;
   uint32_t num_pairs = 0, num_decisions = 0;
@@ -599,7 +565,7 @@ struct TrajBatch {
   std::vector<uint8_t> normalize;
   int32_t* dec = nullptr;
   TrajTilePass* passes = nullptr;
-  uint16_t* recs = nullptr;
+  uint32_t* recs = nullptr;
   unsigned max_tile_bits = 0;  // over the batch's tile passes
   std::vector<uint64_t> state_offset;
   std::vector<unsigned> nq;
@@ -628,8 +594,8 @@ int TileKernelReady() {
       const void* fn;
       size_t lds;
     } fns[2] = {
-        {reinterpret_cast<const void*>(&TrajTileKernel<13>), BlockLds<13>()},
-        {reinterpret_cast<const void*>(&TrajTileKernel<14>), BlockLds<14>()}};
+        {reinterpret_cast<const void*>(&TrajTileKernel<13>), TrajLds<13>()},
+        {reinterpret_cast<const void*>(&TrajTileKernel<14>), TrajLds<14>()}};
     for (const auto& f : fns) {
       hipFuncAttributes fa{};
       if (attr == hipSuccess) attr = hipFuncGetAttributes(&fa, f.fn);
@@ -658,12 +624,12 @@ int Launch(TrajBatch& b, unsigned threads, hipStream_t st) {
       // The 14-qubit shape when a pass needs it.
       if (b.max_tile_bits == 14) {
         hipLaunchKernelGGL(TrajTileKernel<14>, dim3(b.num_traj),
-                           dim3(BlockShape<14>::NT), BlockLds<14>(), st,
+                           dim3(BlockShape<14>::NT), TrajLds<14>(), st,
                            b.descs, b.ops, b.mats, b.passes, b.recs, b.pairs,
                            b.pool, b.zz, b.norms, b.dec);
       } else {
         hipLaunchKernelGGL(TrajTileKernel<13>, dim3(b.num_traj),
-                           dim3(BlockShape<13>::NT), BlockLds<13>(), st,
+                           dim3(BlockShape<13>::NT), TrajLds<13>(), st,
                            b.descs, b.ops, b.mats, b.passes, b.recs, b.pairs,
                            b.pool, b.zz, b.norms, b.dec);
       }
@@ -710,12 +676,12 @@ uint32_t TilePassIndex(unsigned n, const qsim_hip_traj_op* o,
 }
 
 // The device form of a tile pass (header and gates validated, including the
-// conditions PrepareGateBlock checks): geometry, program and op records as
-// the circuit kernel's blocked passes build them, from the gates' FP32
-// matrices (exact in FP64, and exact again in the three bf16 parts); the
-// records go to rec[0 .. k kOpWords 16-bit halves).
+// conditions PrepareGateBlock checks): geometry, program and FP16 op
+// records as the circuit kernel's blocked passes build them, from the
+// gates' FP32 matrices (exact in FP64); the records go to
+// rec[0 .. k kF16RecWords).
 int BuildTilePass(const TilePassJob& job, const float* mats, TrajTilePass& tp,
-                  uint16_t* rec) {
+                  uint32_t* rec) {
   const qsim_hip_traj_op* o = job.o;
   const unsigned n = job.n, nops = o[0].k;
   const uint64_t mask = uint64_t(o[0].q[0]) | (uint64_t(o[0].q[1]) << 32);
@@ -736,9 +702,17 @@ int BuildTilePass(const TilePassJob& job, const float* mats, TrajTilePass& tp,
   }
   std::vector<uint16_t> fb;
   const int rc = PrepareGateBlock(n, tile, nt, nops, ks.data(), qs.data(),
-                                  m64.data(), moff.data(), tp.g, tp.p, fb);
-  if (rc == QSIM_HIP_OK) std::copy(fb.begin(), fb.end(), rec);
-  return rc;
+                                  m64.data(), moff.data(), tp.g, tp.p, fb,
+                                  kBlockArithF16);
+  if (rc != QSIM_HIP_OK) return rc;
+  if (tp.p.arith != kBlockArithF16) {
+    // Matrices that could grow the state by 2^10: not Kraus operators.
+    SetError("tile pass gates grow the state by more than 2^10 (run the "
+             "per-gate kernels)");
+    return QSIM_HIP_ERR_UNSUPPORTED;
+  }
+  std::memcpy(rec, fb.data(), fb.size() * sizeof(uint16_t));
+  return QSIM_HIP_OK;
 }
 
 // Builds the distinct passes, spread over a few host threads that take jobs
@@ -748,8 +722,8 @@ int BuildTilePass(const TilePassJob& job, const float* mats, TrajTilePass& tp,
 // return instead of ending the process.
 int BuildTilePasses(const std::vector<TilePassJob>& jobs, const float* mats,
                     std::vector<TrajTilePass>& passes,
-                    std::vector<uint16_t>& recs) {
-  constexpr size_t kRec = 2 * size_t(kOpWords);  // 16-bit halves per record
+                    std::vector<uint32_t>& recs) {
+  constexpr size_t kRec = kF16RecWords;  // words per record
   passes.assign(jobs.size(), TrajTilePass{});
   size_t total = 0;
   for (size_t j = 0; j < jobs.size(); ++j) {
@@ -905,7 +879,8 @@ static int TrajPrepare(const qsim_hip_traj_desc* trajs, unsigned num_traj,
         // every qubit of the pass's gates.
         const uint64_t mask = uint64_t(o.q[0]) | (uint64_t(o.q[1]) << 32);
         const unsigned nt = unsigned(__builtin_popcountll(mask));
-        ok = o.k >= 1 && o.k <= kBlockMaxOps && i + o.k < d[t].op_end &&
+        // At most as many gates as the tile kernel keeps resident.
+        ok = o.k >= 1 && o.k <= BlockResidentOps(nt) && i + o.k < d[t].op_end &&
              (mask >> n) == 0 && nt >= kBlockMinBits && nt <= kBlockMaxBits &&
              (mask & 7u) == 7u;
         for (unsigned j = 1; ok && j <= o.k; ++j) {
@@ -935,7 +910,7 @@ static int TrajPrepare(const qsim_hip_traj_desc* trajs, unsigned num_traj,
   // index.
   std::vector<qsim_hip_traj_op> ov(ops, ops + num_ops);
   std::vector<TrajTilePass> passes;
-  std::vector<uint16_t> recs;
+  std::vector<uint32_t> recs;
   {
     std::vector<TilePassJob> jobs;
     std::unordered_map<std::string, uint32_t> seen;
