@@ -1,11 +1,11 @@
 // block_tile.h -- the tile machinery of blocked passes, shared by the
-// circuit kernel (block_kernels.hip) and the batched trajectory kernel
-// (traj_kernels.hip): kernel shapes, the op-record layout, the split-BF16
-// MFMA product of one 4-qubit op over an LDS tile, and tile staging between
-// HBM and LDS.
+// circuit kernels (block_kernels.hip) and the batched trajectory kernel
+// (traj_kernels.hip): kernel shapes, the op-record layouts, the MFMA
+// product of one 4-qubit op over an LDS tile (FP16 with scaling, the
+// default, and split-BF16), and tile staging between HBM and LDS.
 //
 // Each op is a 4-qubit gate on tile-local positions (smaller gates are
-// padded with identity factors on the host) applied as the split-BF16 MFMA
+// padded with identity factors on the host).  The split-BF16 form is the
 // product of the gate kernels (mfma_kernels.hip): the 16 x 16 complex
 // matrix becomes a real 32 x 32 A operand, 32 groups side by side the B
 // operand of v_mfma_f32_32x32x16_bf16; the matrix is split in three bf16
