@@ -151,7 +151,7 @@ def test_small_model_trained_on_fresh_timings_plans_well():
 @pytest.mark.parametrize("n,seed", [(28, 101), (30, 102)])
 def test_committed_model_gains_over_fallback(n, seed):
     # The committed model (trained on grid circuits at 24-32 qubits, seeds
-    # 0-2) plans held-out seeds; measured wall time judges it.
+    # 0-2) plans held-out seeds; measured wall time decides.
     m = costmodel.CostModel.load()
     text = circuits.grid_circuit(n, 20, seed=seed)
     c = driver.Circuit(text)
