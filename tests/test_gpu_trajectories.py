@@ -435,6 +435,47 @@ def test_malformed_tile_passes_are_rejected(bad):
     assert not h.value
 
 
+@pytest.mark.parametrize("tile_bits", [13, 14])
+def test_tile_pass_holds_the_resident_op_count(tile_bits):
+    # A pass holds at most as many gates as the tiled kernel keeps resident
+    # in LDS (qsim_hip_block_resident_ops): that many run and agree with the
+    # per-gate kernel, one more is refused.
+    n = 16
+    L = qsim_amd.lib()
+    limit = L.qsim_hip_block_resident_ops(tile_bits)
+    assert limit >= 7
+    mask = (1 << tile_bits) - 1
+    rng = np.random.default_rng(tile_bits)
+    gates = [sorted(rng.choice(tile_bits, 2, replace=False).tolist())
+             for _ in range(limit + 1)]
+    ops, mats = tile_stream(mask, gates[:limit])
+    rc, h = raw_batch(n, ops, mats)
+    assert rc == 0
+    out = []
+    for threads in (TILED, 256):
+        assert L.qsim_hip_traj_run(h, threads, 0) == 0
+        st = np.zeros(2 << n, np.float32)
+        assert L.qsim_hip_traj_state(h, 0, st.ctypes.data) == 0
+        out.append(st)
+    L.qsim_hip_traj_free(h)
+    assert np.linalg.norm(out[0] - out[1]) < 1e-5 * np.linalg.norm(out[1])
+    ops, mats = tile_stream(mask, gates)
+    rc, h = raw_batch(n, ops, mats)
+    assert rc == qsim_amd.ERR_ARG
+    assert not h.value
+
+
+def test_tile_pass_that_could_grow_the_state_is_refused():
+    # Tile passes hold unitary gates and Kraus operators (norm <= 1); gates
+    # that could grow the state by 2^10 or more do not fit the FP16 tile
+    # arithmetic and are refused at prepare time.
+    n = 16
+    ops, mats = tile_stream((1 << 10) - 1, [[3], [4, 9]])
+    rc, h = raw_batch(n, ops, mats * np.float32(4096))
+    assert rc == qsim_amd.ERR_UNSUPPORTED
+    assert not h.value
+
+
 def channel_stream(n, nq=1, kraus=2):
     """A channel decision (QSIM_HIP_TRAJ_CHANNEL) on qubits 0.. nq - 1 with
     `kraus` amplitude-damping-like operators (QSIM_HIP_TRAJ_KRAUS)."""
