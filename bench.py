@@ -198,6 +198,13 @@ def run_path(args, text, name):
         st.free()
     log(f"[{name}] {args.steps} circuits in {total_ms:.1f} ms "
         f"(wall {wall:.3f}s), {len(pass_ms)} passes timed, norm {norm:.9f}")
+    if args.pass_times and len(pass_ms) == npass * args.steps:
+        # Median over the timed circuits of each pass, with its qubits (the
+        # tile qubits of a blocked pass) and fused gates.
+        med = np.median(np.asarray(pass_ms).reshape(args.steps, npass), 0)
+        for i, ((q, _), ms) in enumerate(zip(passes, med)):
+            log(f"[{name}] pass {i}: {float(ms):.3f} ms, qubits {list(q)}, "
+                f"fused gates {block_ops[i] or 1}")
 
     bytes_per_pass = 16.0 * (1 << n)   # every complex64 read and written once
     fams = [kernel_family(circ, i, q, method, n, arith)
@@ -591,6 +598,9 @@ def main():
     ap.add_argument("--trajectories", type=int, default=1024,
                     help="size of the trajectory batch reported beside the "
                          "headline (0: skip)")
+    ap.add_argument("--pass-times", action="store_true",
+                    help="log each pass's median time, qubits and fused "
+                         "gates (stderr)")
     ap.add_argument("--no-grid30", action="store_true",
                     help="skip the 30-qubit fusion sweep and its CPU run")
     ap.add_argument("--no-small", action="store_true",
