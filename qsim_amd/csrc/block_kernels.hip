@@ -9,15 +9,19 @@
 // gate sequence whose qubits all lie in the tile set is exact under this
 // reordering: the tiles are independent subspaces of the gates' action.
 //
-// Each program op is a 4-qubit gate on tile-local positions, applied as a
-// split-BF16 MFMA product (block_tile.h).
+// Each program op is a 4-qubit gate on tile-local positions, applied as an
+// MFMA product (block_tile.h): FP16 with a power-of-two scale per tile
+// (GateBlockF16Kernel, the default; every op record of the pass resident in
+// LDS), or split-BF16 (GateBlockKernel, for programs that could grow the
+// state by more than 2^10 and on request; op records through a two-slot
+// ring).
 //
 // The grid is a tile-stride loop.  Tiles of up to 13 qubits: 256 threads
-// share a tile, and the 64 KiB tile (plus a 16 KiB ring of op records) lets
-// two workgroups share a CU's 160 KiB of LDS, so one workgroup's HBM phase
-// overlaps the other's MFMA phase.  14-qubit tiles: 512 threads, one
-// workgroup per CU (128 + 16 KiB); fewer passes for a circuit, with the
-// overlap coming from the next tile's prefetch alone.
+// share a tile, and the 64 KiB tile plus its op records lets two workgroups
+// share a CU's 160 KiB of LDS, so one workgroup's HBM phase overlaps the
+// other's MFMA phase.  14-qubit tiles: 512 threads, one workgroup per CU;
+// fewer passes for a circuit, with the overlap coming from the next tile's
+// prefetch alone.
 
 #include <hip/hip_runtime.h>
 

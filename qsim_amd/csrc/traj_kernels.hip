@@ -10,17 +10,20 @@
 //
 //   gate      a fused gate (k <= 4, FP32 matrix in LDS) over all 2^(n-k)
 //             groups of the trajectory's state;
-//   channel   a channel whose draw fell past the probability bounds: one
-//             FP64 sweep for the norm and every Kraus operator's
-//             <psi|K^dagger K|psi>, the choice exactly as RunIteration makes
-//             it (lib/qtrajectory.h:330-370), then the chosen operator
-//             applied (scaled to the normalised state where the trajectory
-//             is not unitary).
+//   tile pass a run of gates applied tile by tile through LDS with the
+//             FP16 MFMA product of the circuit kernel's blocked passes (the
+//             tiled kernel; the per-gate kernels run the gates one by one);
+//   channel   a channel whose draw fell past the probability bounds: FP64
+//             sweeps for <psi|K^dagger K|psi> operator by operator (the
+//             first also giving the norm), the choice exactly as
+//             RunIteration makes it (lib/qtrajectory.h:330-370), then the
+//             chosen operator applied (scaled to the normalised state where
+//             the trajectory is not unitary).
 //
 // The Kraus operators that the bounds decide (depolarising Paulis, damping
-// K0 when the draw is below 1 - gamma, any channel's operators whose bound
-// covers the draw) are sampled on the host from the same mt19937(rep)
-// draws and folded into the fused gates.  At the end each
+// K0 when the draw is below its bound, any unitary operator whose
+// cumulative bound covers the draw) are sampled on the host from the same
+// mt19937(rep) draws and folded into the fused gates.  At the end each
 // trajectory writes <Z_a Z_b> of its qubit pairs, FP64-accumulated and
 // divided by the norm (the normalisation of lib/qtrajectory.h:377-383).
 //
