@@ -143,6 +143,8 @@ def kernel_family(circ, i, qubits, method, n, arith="f16"):
             return "GateBf16TileKernel" if low else "GateBf16RegKernel"
         if method == 3 and k >= 5 and n >= 12:
             return "GateF16SetKernel" if low else "GateF16RegKernel"
+        if method == 0 and k >= 5 and n >= 11:
+            return "GateF32RowSetKernel" if low else "GateF32RowRegKernel"
         return "GateMfmaTileKernel" if low else "GateMfmaRegKernel"
     return "GateTileKernel" if low else "GateRegKernel"
 

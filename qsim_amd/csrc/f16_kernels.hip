@@ -1,4 +1,5 @@
-// f16_kernels.hip -- k = 5, 6 gate passes on FP16 MFMA with scaling.
+// f16_kernels.hip -- k = 5, 6 gate passes on FP16 MFMA with scaling, and
+// the exact-FP32 k = 5, 6 passes on the same work split (further below).
 //
 // The pass computes, per 2^k-amplitude group, out = U in as the real
 // product C[2D x 32] = A[2D x 2D] B[2D x 32] of mfma_kernels.hip (D = 2^k,
@@ -313,6 +314,184 @@ GateF16SetKernel(float2* __restrict__ s, SetGeom g,
   }
 }
 
+// ---- Exact FP32 on the same work split ------------------------------------
+//
+// v_mfma_f32_32x32x2f32 takes one member per K step: lane l supplies
+// A[32 rb + l%32][2 t + l/32] (the FP32 fragments of mfma_kernels.hip,
+// frag[(rb D + t) 64 + l]) and component l/32 of member t of group l%32,
+// and its C layout is the FP16 product's, so StoreRows applies.  A wave
+// keeps its row block's D fragment values in registers (64 at k = 6) and
+// reads component h of each member from the set in LDS: the matrix never
+// comes from the cache hierarchy again, which the per-batch fragment
+// reloads of GateMfmaRegKernel / GateMfmaTileKernel at k = 5, 6 do.
+
+template <int K>
+struct F32Frag {
+  float f[F16Dims<K>::D];
+  __device__ __forceinline__ void Load(const float* __restrict__ frag,
+                                       int rb) {
+    const int l = threadIdx.x & 63;
+#pragma unroll
+    for (int t = 0; t < F16Dims<K>::D; ++t)
+      f[t] = frag[(rb * F16Dims<K>::D + t) * 64 + l];
+  }
+};
+
+// This wave's 32 output rows of the batch whose member t, component h,
+// is at b[stride t] (floats).
+template <int K>
+__device__ __forceinline__ v16f RowBlockF32(const F32Frag<K>& a,
+                                            const float* b, int stride) {
+  v16f c = v16f{};
+#pragma unroll
+  for (int t = 0; t < F16Dims<K>::D; ++t)
+    c = __builtin_amdgcn_mfma_f32_32x32x2f32(a.f[t], b[stride * t], c, 0, 0,
+                                             0);
+  return c;
+}
+
+template <typename Store>
+__device__ __forceinline__ void StoreRowsF32(const v16f& c, int rb,
+                                             Store store) {
+  const int h = (threadIdx.x >> 5) & 1;
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+      store(16 * rb + 4 * s + 2 * h + u,
+            float2{c[4 * s + 2 * u], c[4 * s + 2 * u + 1]});
+}
+
+// Every target at qubit >= 5 (sets and copies as GateF16RegKernel).
+template <int K>
+__global__ void __launch_bounds__(256)
+GateF32RowRegKernel(float2* __restrict__ s, GateGeom g,
+                    const float* __restrict__ frag) {
+  using Dm = F16Dims<K>;
+  constexpr int G = Dm::G, D = Dm::D;
+  constexpr int PERT = G * D / 256;
+  __shared__ float2 buf[2][G * D];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int rb = wave % Dm::RB, bsub = wave / Dm::RB;
+  const int j = lane & 31, h = lane >> 5;
+  F32Frag<K> a;
+  a.Load(frag, rb);
+  const uint64_t nsets = g.ngroups / G;
+  const unsigned cg = threadIdx.x % G, cm0 = threadIdx.x / G;
+  constexpr unsigned kMStep = 256 / G;
+  auto fetch = [&](uint64_t set, float2 (&x)[PERT]) {
+    const uint64_t base = InsertZeros(set * G + cg, g.ins, g.nins);
+#pragma unroll
+    for (int r = 0; r < PERT; ++r)
+      x[r] = s[base + MemberOffset(cm0 + kMStep * r, g.tbit, K)];
+  };
+  auto put = [&](const float2 (&x)[PERT], float2* b) {
+#pragma unroll
+    for (int r = 0; r < PERT; ++r) b[(cm0 + kMStep * r) * G + cg] = x[r];
+  };
+  float2 pre[PERT];
+  uint64_t set = blockIdx.x;
+  if (set < nsets) {
+    fetch(set, pre);
+    put(pre, buf[0]);
+  }
+  __syncthreads();
+  int cur = 0;
+  for (; set < nsets; set += gridDim.x) {
+    const uint64_t nxt = set + gridDim.x;
+    if (nxt < nsets) fetch(nxt, pre);
+    const float* b =
+        reinterpret_cast<const float*>(buf[cur] + bsub * 32 + j) + h;
+    const v16f c = RowBlockF32<K>(a, b, 2 * G);
+    const uint64_t base = InsertZeros(set * G + bsub * 32 + j, g.ins, g.nins);
+    StoreRowsF32(c, rb, [&](int m, float2 x) {
+      s[base + MemberOffset(m, g.tbit, K)] = x;
+    });
+    if (nxt < nsets) put(pre, buf[cur ^ 1]);
+    __syncthreads();
+    cur ^= 1;
+  }
+}
+
+// A target below qubit 5 (sets, slots and copies as GateF16SetKernel).
+template <int K>
+__global__ void __launch_bounds__(256)
+GateF32RowSetKernel(float2* __restrict__ s, SetGeom g,
+                    const float* __restrict__ frag) {
+  using Dm = F16Dims<K>;
+  constexpr int G = Dm::G, D = Dm::D;
+  constexpr int PERT = 2048 / 256;
+  __shared__ float2 buf[2][2048];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int rb = wave % Dm::RB, bsub = wave / Dm::RB;
+  const int j = lane & 31, h = lane >> 5;
+  F32Frag<K> a;
+  a.Load(frag, rb);
+  auto swz = [](unsigned m, unsigned grp) {
+    return m * unsigned(G) + (grp ^ (m & 31u));
+  };
+  uint64_t goff[PERT];
+  unsigned slot[PERT];
+#pragma unroll
+  for (int r = 0; r < PERT; ++r) {
+    const unsigned x = threadIdx.x + 256u * r;
+    uint64_t o = 0;
+    unsigned m = 0, grp = 0, mi = 0, gi = 0;
+    for (int b = 0; b < 11; ++b) {
+      const unsigned bit = (x >> b) & 1u;
+      o |= uint64_t(bit) << g.pos[b];
+      if ((g.tmask >> b) & 1u) {
+        m |= bit << mi++;
+      } else {
+        grp |= bit << gi++;
+      }
+    }
+    goff[r] = o;
+    slot[r] = swz(m, grp);
+  }
+  float2 pre[PERT];
+  auto fetch = [&](uint64_t set) {
+    const float2* p = s + InsertZeros(set, g.pos, 11);
+#pragma unroll
+    for (int r = 0; r < PERT; ++r) pre[r] = p[goff[r]];
+  };
+  auto put = [&](float2* b) {
+#pragma unroll
+    for (int r = 0; r < PERT; ++r) b[slot[r]] = pre[r];
+  };
+  uint64_t set = blockIdx.x;
+  if (set < g.nsets) {
+    fetch(set);
+    put(buf[0]);
+  }
+  __syncthreads();
+  int cur = 0;
+  const unsigned grp = unsigned(bsub) * 32u + unsigned(j);
+  for (; set < g.nsets; set += gridDim.x) {
+    const uint64_t nxt = set + gridDim.x;
+    if (nxt < g.nsets) fetch(nxt);
+    float2* b = buf[cur];
+    const float* bf = reinterpret_cast<const float*>(b);
+    v16f c = v16f{};
+#pragma unroll
+    for (int t = 0; t < D; ++t)
+      c = __builtin_amdgcn_mfma_f32_32x32x2f32(
+          a.f[t], bf[2 * swz(unsigned(t), grp) + h], c, 0, 0, 0);
+    // Every wave has read the set before any result overwrites it.
+    __syncthreads();
+    StoreRowsF32(c, rb, [&](int m, float2 x) { b[swz(m, grp)] = x; });
+    __syncthreads();
+    float2* p = s + InsertZeros(set, g.pos, 11);
+#pragma unroll
+    for (int r = 0; r < PERT; ++r) p[goff[r]] = b[slot[r]];
+    if (nxt < g.nsets) put(buf[cur ^ 1]);
+    __syncthreads();
+    cur ^= 1;
+  }
+}
+
 // Host: the fp16 (high, low) fragments of A = the real embedding of
 // 2^-sm... scaled matrix, and the scale exponent sm (the kernels multiply
 // their results by 2^-(e + sm)).  frag: RB x KB x 2 x 64 lanes x 8 halves.
@@ -350,6 +529,63 @@ int BuildFragF16(const double* m, std::vector<uint16_t>& frag) {
   return sm;
 }
 
+// The set's qubits: the targets and the 11 - k lowest other qubits; false
+// when the state has fewer than 11 qubits.
+bool MakeSetGeom(unsigned n, const unsigned* qs, int K, SetGeom& g) {
+  if (n < 11) return false;
+  g = SetGeom{};
+  g.nsets = uint64_t{1} << (n - 11);
+  unsigned np = 0, others = 0, ti = 0;
+  for (unsigned q = 0; q < n && np < 11; ++q) {
+    const bool target = ti < unsigned(K) && qs[ti] == q;
+    if (target) {
+      g.tmask |= 1u << np;
+      g.pos[np++] = q;
+      ++ti;
+    } else if (others < 11u - unsigned(K)) {
+      g.pos[np++] = q;
+      ++others;
+    }
+  }
+  return np == 11 && ti == unsigned(K);
+}
+
+template <int K>
+int DispatchF32Rows(float2* s, unsigned n, const unsigned* qs,
+                    const float* frag) {
+  using Dm = F16Dims<K>;
+  if (qs[0] >= 5) {
+    GateGeom g{};
+    g.nins = K;
+    for (int b = 0; b < K; ++b) {
+      g.ins[b] = qs[b];
+      g.tbit[b] = uint64_t{1} << qs[b];
+    }
+    g.ngroups = uint64_t{1} << (n - K);
+    g.cbits = 0;
+    if (g.ngroups < uint64_t(Dm::G)) return QSIM_HIP_ERR_UNSUPPORTED;
+    const uint64_t nsets = g.ngroups / Dm::G;
+    const uint64_t blocks =
+        std::max<uint64_t>(1, std::min<uint64_t>(nsets, uint64_t(NumCUs()) * 4));
+    PassTimerBegin();
+    hipLaunchKernelGGL(GateF32RowRegKernel<K>, dim3(unsigned(blocks)),
+                       dim3(256), 0, Stream(), s, g, frag);
+    PassTimerEnd();
+    QHIP_LAUNCHED("GateF32RowRegKernel");
+    return QSIM_HIP_OK;
+  }
+  SetGeom g{};
+  if (!MakeSetGeom(n, qs, K, g)) return QSIM_HIP_ERR_UNSUPPORTED;
+  const uint64_t blocks = std::max<uint64_t>(
+      1, std::min<uint64_t>(g.nsets, uint64_t(NumCUs()) * 4));
+  PassTimerBegin();
+  hipLaunchKernelGGL(GateF32RowSetKernel<K>, dim3(unsigned(blocks)),
+                     dim3(256), 0, Stream(), s, g, frag);
+  PassTimerEnd();
+  QHIP_LAUNCHED("GateF32RowSetKernel");
+  return QSIM_HIP_OK;
+}
+
 template <int K>
 int DispatchF16(float2* s, unsigned n, const unsigned* qs, const double* m64) {
   using Dm = F16Dims<K>;
@@ -382,23 +618,8 @@ int DispatchF16(float2* s, unsigned n, const unsigned* qs, const double* m64) {
     return QSIM_HIP_OK;
   }
 
-  // The set's qubits: the targets and the 11 - k lowest other qubits.
-  if (n < 11) return QSIM_HIP_ERR_UNSUPPORTED;
   SetGeom g{};
-  g.nsets = uint64_t{1} << (n - 11);
-  unsigned np = 0, others = 0, ti = 0;
-  for (unsigned q = 0; q < n && np < 11; ++q) {
-    const bool target = ti < unsigned(K) && qs[ti] == q;
-    if (target) {
-      g.tmask |= 1u << np;
-      g.pos[np++] = q;
-      ++ti;
-    } else if (others < 11u - K) {
-      g.pos[np++] = q;
-      ++others;
-    }
-  }
-  if (np != 11 || ti != unsigned(K)) return QSIM_HIP_ERR_UNSUPPORTED;
+  if (!MakeSetGeom(n, qs, K, g)) return QSIM_HIP_ERR_UNSUPPORTED;
   const uint64_t blocks = std::max<uint64_t>(
       1, std::min<uint64_t>(g.nsets, uint64_t(NumCUs()) * 4));
   PassTimerBegin();
@@ -421,6 +642,13 @@ int LaunchGateF16(float2* s, unsigned n, const unsigned* qs, unsigned k,
     }
   }
   return k == 5 ? DispatchF16<5>(s, n, qs, m64) : DispatchF16<6>(s, n, qs, m64);
+}
+
+int LaunchRowsF32(float2* s, unsigned n, const unsigned* qs, unsigned k,
+                  const float* frag) {
+  if (k < 5 || k > 6) return QSIM_HIP_ERR_UNSUPPORTED;
+  return k == 5 ? DispatchF32Rows<5>(s, n, qs, frag)
+                : DispatchF32Rows<6>(s, n, qs, frag);
 }
 
 }  // namespace qhip

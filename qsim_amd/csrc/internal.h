@@ -231,6 +231,13 @@ int LaunchGateMfma(float2* s, unsigned n, const unsigned* qs, unsigned k,
 int LaunchGateF16(float2* s, unsigned n, const unsigned* qs, unsigned k,
                   const double* m64);
 
+// Exact-FP32 MFMA for uncontrolled k = 5, 6 gates with each wave's row
+// block of the matrix held in registers (f16_kernels.hip); `frag`: the
+// device FP32 fragments of mfma_kernels.hip.  QSIM_HIP_ERR_UNSUPPORTED when
+// the state is too small for its sets.
+int LaunchRowsF32(float2* s, unsigned n, const unsigned* qs, unsigned k,
+                  const float* frag);
+
 // Split-BF16 A-operand fragments of a 2^K x 2^K complex matrix (FP64,
 // row-major interleaved): 2^K/16 row blocks x 2^(K+1)/16 K blocks x
 // (hi, mid, lo) x 512.
