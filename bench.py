@@ -140,6 +140,8 @@ def kernel_family(circ, i, qubits, method, n, arith="f16"):
     low = bool(qubits) and min(qubits) < 5
     if method != 1 and 4 <= k <= 6 and n >= k + 5:
         if method == 2 and k >= 5:
+            if n >= 11:
+                return "GateBf16RowSetKernel" if low else "GateBf16RowRegKernel"
             return "GateBf16TileKernel" if low else "GateBf16RegKernel"
         if method == 3 and k >= 5 and n >= 12:
             return "GateF16SetKernel" if low else "GateF16RegKernel"

@@ -1,5 +1,6 @@
 // f16_kernels.hip -- k = 5, 6 gate passes on FP16 MFMA with scaling, and
-// the exact-FP32 k = 5, 6 passes on the same work split (further below).
+// the exact-FP32 and split-BF16 k = 5, 6 passes on the same work split
+// (further below).
 //
 // The pass computes, per 2^k-amplitude group, out = U in as the real
 // product C[2D x 32] = A[2D x 2D] B[2D x 32] of mfma_kernels.hip (D = 2^k,
@@ -492,6 +493,185 @@ GateF32RowSetKernel(float2* __restrict__ s, SetGeom g,
   }
 }
 
+// ---- Split-BF16 on the same work split --------------------------------------
+//
+// v_mfma_f32_32x32x16_bf16 has the FP16 product's operand maps (lane half h
+// supplies members 8 kb + 4 h + q of K block kb).  A wave keeps its row
+// block's three bf16 parts (hi, mid, lo per K block: 96 registers at
+// k = 6) and splits each K block of amplitudes in two (hi, lo) as it reads
+// them from the set in LDS; Al Bh + Am Bh + Ah Bl + Ah Bh per K block, as
+// the split-BF16 kernels of mfma_kernels.hip accumulate them.
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+template <int K>
+struct Bf16Frag {
+  v8bf f[F16Dims<K>::KB][3];
+  __device__ __forceinline__ void Load(const uint4* __restrict__ frag,
+                                       int rb) {
+    const int l = threadIdx.x & 63;
+#pragma unroll
+    for (int kb = 0; kb < F16Dims<K>::KB; ++kb)
+#pragma unroll
+      for (int p = 0; p < 3; ++p)
+        f[kb][p] = __builtin_bit_cast(
+            v8bf, frag[((rb * F16Dims<K>::KB + kb) * 3 + p) * 64 + l]);
+  }
+};
+
+// This wave's 32 output rows; read(t) gives the lane's member
+// 8 (t / 4) + 4 h + t % 4 of its group.
+template <int K, typename Read>
+__device__ __forceinline__ v16f RowBlockBf16(const Bf16Frag<K>& a,
+                                             Read read) {
+  v16f c = v16f{};
+#pragma unroll
+  for (int kb = 0; kb < F16Dims<K>::KB; ++kb) {
+    v8bf bh, bl;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float2 v = read(4 * kb + q);
+      const __bf16 hr = (__bf16)v.x, hi = (__bf16)v.y;
+      bh[2 * q] = hr;
+      bh[2 * q + 1] = hi;
+      bl[2 * q] = (__bf16)(v.x - (float)hr);
+      bl[2 * q + 1] = (__bf16)(v.y - (float)hi);
+    }
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.f[kb][2], bh, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.f[kb][1], bh, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.f[kb][0], bl, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.f[kb][0], bh, c, 0, 0, 0);
+  }
+  return c;
+}
+
+// Every target at qubit >= 5 (sets and copies as GateF16RegKernel).
+template <int K>
+__global__ void __launch_bounds__(256)
+GateBf16RowRegKernel(float2* __restrict__ s, GateGeom g,
+                     const uint4* __restrict__ frag) {
+  using Dm = F16Dims<K>;
+  constexpr int G = Dm::G, D = Dm::D;
+  constexpr int PERT = G * D / 256;
+  __shared__ float2 buf[2][G * D];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int rb = wave % Dm::RB, bsub = wave / Dm::RB;
+  const int j = lane & 31, h = lane >> 5;
+  Bf16Frag<K> a;
+  a.Load(frag, rb);
+  const uint64_t nsets = g.ngroups / G;
+  const unsigned cg = threadIdx.x % G, cm0 = threadIdx.x / G;
+  constexpr unsigned kMStep = 256 / G;
+  auto fetch = [&](uint64_t set, float2 (&x)[PERT]) {
+    const uint64_t base = InsertZeros(set * G + cg, g.ins, g.nins);
+#pragma unroll
+    for (int r = 0; r < PERT; ++r)
+      x[r] = s[base + MemberOffset(cm0 + kMStep * r, g.tbit, K)];
+  };
+  auto put = [&](const float2 (&x)[PERT], float2* b) {
+#pragma unroll
+    for (int r = 0; r < PERT; ++r) b[(cm0 + kMStep * r) * G + cg] = x[r];
+  };
+  float2 pre[PERT];
+  uint64_t set = blockIdx.x;
+  if (set < nsets) {
+    fetch(set, pre);
+    put(pre, buf[0]);
+  }
+  __syncthreads();
+  int cur = 0;
+  for (; set < nsets; set += gridDim.x) {
+    const uint64_t nxt = set + gridDim.x;
+    if (nxt < nsets) fetch(nxt, pre);
+    const float2* b = buf[cur] + bsub * 32 + j;
+    const v16f c = RowBlockBf16<K>(
+        a, [&](int t) { return b[(8 * (t >> 2) + 4 * h + (t & 3)) * G]; });
+    const uint64_t base = InsertZeros(set * G + bsub * 32 + j, g.ins, g.nins);
+    StoreRowsF32(c, rb, [&](int m, float2 x) {
+      s[base + MemberOffset(m, g.tbit, K)] = x;
+    });
+    if (nxt < nsets) put(pre, buf[cur ^ 1]);
+    __syncthreads();
+    cur ^= 1;
+  }
+}
+
+// A target below qubit 5 (sets, slots and copies as GateF16SetKernel).
+template <int K>
+__global__ void __launch_bounds__(256)
+GateBf16RowSetKernel(float2* __restrict__ s, SetGeom g,
+                     const uint4* __restrict__ frag) {
+  using Dm = F16Dims<K>;
+  constexpr int G = Dm::G;
+  constexpr int PERT = 2048 / 256;
+  __shared__ float2 buf[2][2048];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int rb = wave % Dm::RB, bsub = wave / Dm::RB;
+  const int j = lane & 31, h = lane >> 5;
+  Bf16Frag<K> a;
+  a.Load(frag, rb);
+  auto swz = [](unsigned m, unsigned grp) {
+    return m * unsigned(G) + (grp ^ (m & 31u));
+  };
+  uint64_t goff[PERT];
+  unsigned slot[PERT];
+#pragma unroll
+  for (int r = 0; r < PERT; ++r) {
+    const unsigned x = threadIdx.x + 256u * r;
+    uint64_t o = 0;
+    unsigned m = 0, grp = 0, mi = 0, gi = 0;
+    for (int b = 0; b < 11; ++b) {
+      const unsigned bit = (x >> b) & 1u;
+      o |= uint64_t(bit) << g.pos[b];
+      if ((g.tmask >> b) & 1u) {
+        m |= bit << mi++;
+      } else {
+        grp |= bit << gi++;
+      }
+    }
+    goff[r] = o;
+    slot[r] = swz(m, grp);
+  }
+  float2 pre[PERT];
+  auto fetch = [&](uint64_t set) {
+    const float2* p = s + InsertZeros(set, g.pos, 11);
+#pragma unroll
+    for (int r = 0; r < PERT; ++r) pre[r] = p[goff[r]];
+  };
+  auto put = [&](float2* b) {
+#pragma unroll
+    for (int r = 0; r < PERT; ++r) b[slot[r]] = pre[r];
+  };
+  uint64_t set = blockIdx.x;
+  if (set < g.nsets) {
+    fetch(set);
+    put(buf[0]);
+  }
+  __syncthreads();
+  int cur = 0;
+  const unsigned grp = unsigned(bsub) * 32u + unsigned(j);
+  for (; set < g.nsets; set += gridDim.x) {
+    const uint64_t nxt = set + gridDim.x;
+    if (nxt < g.nsets) fetch(nxt);
+    float2* b = buf[cur];
+    const v16f c = RowBlockBf16<K>(a, [&](int t) {
+      return b[swz(8u * unsigned(t >> 2) + 4u * unsigned(h) + unsigned(t & 3),
+                   grp)];
+    });
+    // Every wave has read the set before any result overwrites it.
+    __syncthreads();
+    StoreRowsF32(c, rb, [&](int m, float2 x) { b[swz(m, grp)] = x; });
+    __syncthreads();
+    float2* p = s + InsertZeros(set, g.pos, 11);
+#pragma unroll
+    for (int r = 0; r < PERT; ++r) p[goff[r]] = b[slot[r]];
+    if (nxt < g.nsets) put(buf[cur ^ 1]);
+    __syncthreads();
+    cur ^= 1;
+  }
+}
+
 // Host: the fp16 (high, low) fragments of A = the real embedding of
 // 2^-sm... scaled matrix, and the scale exponent sm (the kernels multiply
 // their results by 2^-(e + sm)).  frag: RB x KB x 2 x 64 lanes x 8 halves.
@@ -587,6 +767,42 @@ int DispatchF32Rows(float2* s, unsigned n, const unsigned* qs,
 }
 
 template <int K>
+int DispatchBf16Rows(float2* s, unsigned n, const unsigned* qs,
+                     const uint4* frag) {
+  using Dm = F16Dims<K>;
+  if (qs[0] >= 5) {
+    GateGeom g{};
+    g.nins = K;
+    for (int b = 0; b < K; ++b) {
+      g.ins[b] = qs[b];
+      g.tbit[b] = uint64_t{1} << qs[b];
+    }
+    g.ngroups = uint64_t{1} << (n - K);
+    g.cbits = 0;
+    if (g.ngroups < uint64_t(Dm::G)) return QSIM_HIP_ERR_UNSUPPORTED;
+    const uint64_t nsets = g.ngroups / Dm::G;
+    const uint64_t blocks =
+        std::max<uint64_t>(1, std::min<uint64_t>(nsets, uint64_t(NumCUs()) * 4));
+    PassTimerBegin();
+    hipLaunchKernelGGL(GateBf16RowRegKernel<K>, dim3(unsigned(blocks)),
+                       dim3(256), 0, Stream(), s, g, frag);
+    PassTimerEnd();
+    QHIP_LAUNCHED("GateBf16RowRegKernel");
+    return QSIM_HIP_OK;
+  }
+  SetGeom g{};
+  if (!MakeSetGeom(n, qs, K, g)) return QSIM_HIP_ERR_UNSUPPORTED;
+  const uint64_t blocks = std::max<uint64_t>(
+      1, std::min<uint64_t>(g.nsets, uint64_t(NumCUs()) * 4));
+  PassTimerBegin();
+  hipLaunchKernelGGL(GateBf16RowSetKernel<K>, dim3(unsigned(blocks)),
+                     dim3(256), 0, Stream(), s, g, frag);
+  PassTimerEnd();
+  QHIP_LAUNCHED("GateBf16RowSetKernel");
+  return QSIM_HIP_OK;
+}
+
+template <int K>
 int DispatchF16(float2* s, unsigned n, const unsigned* qs, const double* m64) {
   using Dm = F16Dims<K>;
   std::vector<uint16_t> frag;
@@ -649,6 +865,14 @@ int LaunchRowsF32(float2* s, unsigned n, const unsigned* qs, unsigned k,
   if (k < 5 || k > 6) return QSIM_HIP_ERR_UNSUPPORTED;
   return k == 5 ? DispatchF32Rows<5>(s, n, qs, frag)
                 : DispatchF32Rows<6>(s, n, qs, frag);
+}
+
+int LaunchRowsBf16(float2* s, unsigned n, const unsigned* qs, unsigned k,
+                   const uint16_t* frag) {
+  if (k < 5 || k > 6) return QSIM_HIP_ERR_UNSUPPORTED;
+  const uint4* f4 = reinterpret_cast<const uint4*>(frag);
+  return k == 5 ? DispatchBf16Rows<5>(s, n, qs, f4)
+                : DispatchBf16Rows<6>(s, n, qs, f4);
 }
 
 }  // namespace qhip

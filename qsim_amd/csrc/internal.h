@@ -237,6 +237,9 @@ int LaunchGateF16(float2* s, unsigned n, const unsigned* qs, unsigned k,
 // the state is too small for its sets.
 int LaunchRowsF32(float2* s, unsigned n, const unsigned* qs, unsigned k,
                   const float* frag);
+// The same for the split-BF16 product; `frag`: BuildFragBf16's layout.
+int LaunchRowsBf16(float2* s, unsigned n, const unsigned* qs, unsigned k,
+                   const uint16_t* frag);
 
 // Split-BF16 A-operand fragments of a 2^K x 2^K complex matrix (FP64,
 // row-major interleaved): 2^K/16 row blocks x 2^(K+1)/16 K blocks x

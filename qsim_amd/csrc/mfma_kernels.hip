@@ -412,11 +412,15 @@ int DispatchMfma(float2* s, unsigned n, const unsigned* qs, const double* m64,
   QHIP_TRY(StageMatrix(dfrag, bf16 ? (const void*)fb.data()
                                    : (const void*)f32.data(),
                        fbytes));
-  // Exact FP32 at k = 5, 6: each wave keeps its row block of the matrix in
-  // registers (no per-batch fragment reloads through the cache).
-  if (!bf16 && K >= 5) {
-    const int rc = LaunchRowsF32(s, n, qs, unsigned(K),
-                                 static_cast<const float*>(dfrag));
+  // k = 5, 6 (exact FP32 or split-BF16): each wave keeps its row block of
+  // the matrix in registers (no per-batch fragment reloads through the
+  // cache).
+  if (K >= 5) {
+    const int rc =
+        bf16 ? LaunchRowsBf16(s, n, qs, unsigned(K),
+                              static_cast<const uint16_t*>(dfrag))
+             : LaunchRowsF32(s, n, qs, unsigned(K),
+                             static_cast<const float*>(dfrag));
     if (rc != QSIM_HIP_ERR_UNSUPPORTED) return rc;
   }
 
