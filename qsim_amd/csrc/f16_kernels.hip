@@ -51,7 +51,9 @@ namespace qhip {
 namespace {
 
 typedef float v16f __attribute__((ext_vector_type(16)));
+typedef float v2f __attribute__((ext_vector_type(2)));
 typedef _Float16 v8h __attribute__((ext_vector_type(8)));
+typedef _Float16 v2h __attribute__((ext_vector_type(2)));
 
 template <int K>
 struct F16Dims {
@@ -120,12 +122,23 @@ __device__ __forceinline__ v16f RowBlock(const F16Frag<K>& a, Load load,
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const float2 v = load(4 * kb + q);
-      const float re = ldexpf(v.x, e), im = ldexpf(v.y, e);
-      const _Float16 hr = (_Float16)re, hi = (_Float16)im;
-      bh[2 * q] = hr;
-      bh[2 * q + 1] = hi;
-      bl[2 * q] = (_Float16)(re - (float)hr);
-      bl[2 * q + 1] = (_Float16)(im - (float)hi);
+      const v2f x = v2f{ldexpf(v.x, e), ldexpf(v.y, e)};
+      // High parts rounded to nearest (one packed conversion); the
+      // residuals x - hi are exact in FP32 and v_fma_mix{lo,hi}_f16 round
+      // them to fp16 straight from the fp16 high half.
+      const v2h hi = __builtin_convertvector(x, v2h);
+      uint32_t lw;
+      asm("v_fma_mixlo_f16 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]"
+          : "=v"(lw)
+          : "v"(hi), "v"(x[0]));
+      asm("v_fma_mixhi_f16 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]"
+          : "+v"(lw)
+          : "v"(hi), "v"(x[1]));
+      const v2h lo = __builtin_bit_cast(v2h, lw);
+      bh[2 * q] = hi[0];
+      bh[2 * q + 1] = hi[1];
+      bl[2 * q] = lo[0];
+      bl[2 * q + 1] = lo[1];
     }
     c = __builtin_amdgcn_mfma_f32_32x32x16_f16(a.f[kb][1], bh, c, 0, 0, 0);
     c = __builtin_amdgcn_mfma_f32_32x32x16_f16(a.f[kb][0], bl, c, 0, 0, 0);
