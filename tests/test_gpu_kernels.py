@@ -464,6 +464,36 @@ def test_fp16x2_mfma_accuracy(sim):
         qsim_amd.set_gate_method(qsim_amd.METHOD_AUTO)
 
 
+@pytest.mark.parametrize("method,bar", [
+    (qsim_amd.METHOD_AUTO, 1e-6),           # exact FP32, row blocks
+    (qsim_amd.METHOD_MFMA_BF16X3, 5e-5),    # split-BF16, row blocks
+    (qsim_amd.METHOD_MFMA_FP16X2, 1e-6)])   # FP16 with scaling
+def test_k56_row_block_kernels_over_many_sets(sim, method, bar):
+    # At 22 qubits every k = 5, 6 kernel has more sets of groups than
+    # workgroups (2048 against at most 4 per CU), so each workgroup runs
+    # its double-buffered loop: the next set's loads in flight while the
+    # current one is computed, for low (LDS set) and high (register)
+    # targets.
+    rng = np.random.default_rng(24)
+    n = 22
+    qsim_amd.set_gate_method(method)
+    try:
+        for k in (5, 6):
+            for qs in ([0, 1, 2, 3, 4, 5][:k], [3, 7, 11, 15, 19, 21][:k],
+                       [5, 9, 12, 14, 17, 20][:k]):
+                u = random_matrix(k, rng)
+                s = random_state(n, rng, np.float64)
+                st = upload(s.astype(np.float32), n, np.float32)
+                sim.apply_gate(qs, to_interleaved(u, np.float64), st)
+                oracle.apply_gate(n, s, qs, to_interleaved(u, np.float64))
+                err = rel_l2(st.to_numpy(), s)
+                print(f"method {method} k={k} qs={qs} relL2 {err:.2e}",
+                      flush=True)
+                assert err < bar, (method, k, qs, err)
+    finally:
+        qsim_amd.set_gate_method(qsim_amd.METHOD_AUTO)
+
+
 @pytest.mark.parametrize("scale", [1e-30, 1e-3, 1.0, 1e3, 1e30])
 def test_fp16x2_scaling_over_magnitudes(sim, scale):
     # The per-group scale keeps fp16's 5-bit exponent out of the way: groups
